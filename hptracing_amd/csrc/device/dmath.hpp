@@ -1,0 +1,107 @@
+// Device arithmetic of the path tracer.  Every operation keeps the order
+// written in src/hp/cl_src/kernel.cl, with the OpenCL builtins spelled out:
+//   dot(a,b)   = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   cross(a,b) = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)
+//   length(a)  = sqrt(dot(a,a)), normalize(a) = a / length(a) per component
+// The library is compiled with -ffp-contract=off and correctly rounded
+// division and square root, so +,-,*,/ and sqrt round exactly as IEEE
+// float32 does on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hpt {
+
+struct V3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 xyz(float4 a) { return V3{a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 mulv(V3 a, V3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ V3 scl(float s, V3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length3(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalize3(V3 a) {
+    const float l = length3(a);
+    return mk(a.x / l, a.y / l, a.z / l);
+}
+
+constexpr float kPi = 3.1415926535f;            // PI, kernel.cl:27
+constexpr float kGeneralThreshold = 1e-4f;      // kernel.cl:49
+constexpr float kStepEpsilon = 1e-5f;           // kernel.cl:411
+constexpr int kMaxMatchData = 512;              // kernel.cl:133
+
+// Slab test `_box_intersect` (kernel.cl:51-83): a start point inside the
+// (closed) box hits; otherwise per-axis quotients (division, not a
+// reciprocal), IEEE minNum/maxNum, reject when the nearest exit is <= 0.
+__device__ __forceinline__ bool box_hit(V3 bs, V3 be, V3 sp, V3 d) {
+    if (sp.x >= bs.x && sp.x <= be.x && sp.y >= bs.y && sp.y <= be.y && sp.z >= bs.z &&
+        sp.z <= be.z)
+        return true;
+    const float tx0 = (bs.x - sp.x) / d.x, tx1 = (be.x - sp.x) / d.x;
+    const float ty0 = (bs.y - sp.y) / d.y, ty1 = (be.y - sp.y) / d.y;
+    const float tz0 = (bs.z - sp.z) / d.z, tz1 = (be.z - sp.z) / d.z;
+    const float max_of_mins = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    const float min_of_maxs = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    if (min_of_maxs <= 0) return false;
+    return max_of_mins <= min_of_maxs;
+}
+
+// Moller-Trumbore `_single_intersect` (kernel.cl:85-109) on a triangle
+// record that stores pa and the edges e1 = pb - pa, e2 = pc - pa (the
+// same float subtractions the kernel performs).  The degenerate guard of
+// the reference can never fire, so none is applied.
+__device__ __forceinline__ float tri_hit(V3 sp, V3 d, V3 pa, V3 e1, V3 e2) {
+    const V3 P = cross(d, e2);
+    const float det = dot(e1, P);
+    const float inv_det = 1.f / det;
+    const V3 T = sub(sp, pa);
+    const float u = dot(T, P) * inv_det;
+    if (u < 0.f || u > 1.f) return -1.f;
+    const V3 Q = cross(T, e1);
+    const float v = dot(d, Q) * inv_det;
+    if (v < 0.f || u + v > 1.f) return -1.f;
+    const float t = dot(e2, Q) * inv_det;
+    if (t > 0) return t;
+    return -1.f;
+}
+
+// ------------------------------------------------------------------ RNG
+// 48-bit LCG of kernel.cl:29-47.
+__device__ __forceinline__ uint32_t lcg_rand(uint64_t &s) {
+    s = (s * 0x5DEECE66DULL + 0xBULL) & ((1ULL << 48) - 1);
+    return (uint32_t)(s >> 16);
+}
+__device__ __forceinline__ float lcg_randf(uint64_t &s) {
+    return (float)lcg_rand(s) / 4294967296.0f;  // float(0xFFFFFFFF) rounds to 2^32
+}
+__device__ __forceinline__ V3 lcg_randf3(uint64_t &s) {
+    const float alpha = lcg_randf(s) * kPi;
+    const float beta = lcg_randf(s) * kPi;
+    const float cos_beta = cosf(beta);
+    return mk(cos_beta * cosf(alpha), cos_beta * sinf(alpha), sinf(beta));
+}
+
+__host__ __device__ __forceinline__ uint64_t splitmix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+// Initial LCG state of camera ray `ray` in pass `pass` of a render seeded
+// with `seed`: every path owns its random stream, so a pixel's result does
+// not depend on which other rays share the launch.
+__host__ __device__ __forceinline__ uint64_t path_seed(uint64_t seed, int64_t ray, int32_t pass) {
+    return splitmix(splitmix(seed ^ splitmix((uint64_t)ray)) + (uint64_t)(uint32_t)pass) &
+           ((1ULL << 48) - 1);
+}
+
+}  // namespace hpt

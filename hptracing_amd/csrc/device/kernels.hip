@@ -1,0 +1,585 @@
+// HIP kernels of the wavefront path tracer for gfx950 (CDNA4, wave64).
+//
+//   k_cast     KD traversal + triangle intersection   (kdtree_intersect, kernel.cl:111-208)
+//   k_camera   pinhole / equirectangular camera rays  (set_viewdirs[_vr], kernel.cl:559-613)
+//   k_primary  primary-hit cache and background       (kernel.cl:187-207, trace_runner.cpp:192-213)
+//   k_expand   per-pass restore of the cached hits    (trace_runner.cpp:223-229)
+//   k_shade    emission, interpolation, texture, branch choice and the four
+//              bounce rules fused, with stream compaction of the survivors
+//              (s1_run + s2_*_run, kernel.cl:258-557)
+//   k_resolve  box filter, tone map, UNORM8           (get_image, kernel.cl:615-652)
+//
+// The cast and shade kernels are persistent: a grid sized to the machine
+// pulls work until the device-side live count is exhausted, so the depth
+// loop needs no host round trip and can be captured in a hipGraph.
+#include "dmath.hpp"
+#include "kernels.hpp"
+#include "launch.hpp"
+
+namespace hpt {
+
+// ------------------------------------------------------------------ cast
+// One ray of kdtree_intersect.  The node walk is the reference's stackless
+// pre-order walk expressed with escape indices: a box hit on an interior node
+// descends to idx + 1, anything else continues at the node's escape index
+// (idx + 1 for a leaf).  Leaves are tested the moment they are reached,
+// which is the order the reference tests its collected list in; the walk
+// stops after the 512th non-empty leaf because the reference collects no
+// more (kernel.cl:146) and so can change nothing after it.
+template <bool COUNT>
+__device__ __forceinline__ int trace_ray(const DevScene &sc, V3 sp, V3 d, float &t_out,
+                                         int *cnt) {
+    int idx = 0, leaves = 0, best_id = -1;
+    float best = -42.f;
+    int visits = 0, tested = 0, dropped = 0, visits_cap = -1;
+    const int nn = sc.n_nodes;
+    while (idx < nn) {
+        const float4 lo = sc.nodes[2 * idx];
+        const float4 hi = sc.nodes[2 * idx + 1];
+        const int a = __float_as_int(lo.w);
+        const int b = __float_as_int(hi.w);
+        if (COUNT) ++visits;
+        const bool hit = box_hit(xyz(lo), xyz(hi), sp, d);
+        if (b < 0) {
+            idx = hit ? idx + 1 : a;
+            continue;
+        }
+        if (hit && a > 0) {
+            if (!COUNT || leaves < kMaxMatchData) {
+                const float4 *rec = sc.tris + 3 * (int64_t)b;
+                for (int k = 0; k < a; ++k) {
+                    const float4 r0 = rec[3 * k], r1 = rec[3 * k + 1], r2 = rec[3 * k + 2];
+                    const float r = tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
+                    if (r > 0 && (best < 0 || r < best)) {
+                        best = r;
+                        best_id = __float_as_int(r0.w);
+                    }
+                }
+                if (COUNT) tested += a;
+                ++leaves;
+                if (leaves == kMaxMatchData) {
+                    if (!COUNT) break;
+                    visits_cap = visits;
+                }
+            } else {
+                ++dropped;
+            }
+        }
+        ++idx;
+    }
+    if (COUNT) {
+        cnt[0] = visits_cap >= 0 ? visits_cap : visits;
+        cnt[1] = leaves;
+        cnt[2] = tested;
+        cnt[3] = dropped;
+        cnt[4] = visits;
+    }
+    t_out = best;
+    return best_id;
+}
+
+__device__ __forceinline__ void wave_done(WorkCounter *wc) {
+    if ((threadIdx.x & 63) == 0) {
+        const int total_waves = (int)(gridDim.x * (blockDim.x / 64));
+        __threadfence();
+        const int prev = atomicAdd(&wc->done, 1);
+        if (prev == total_waves - 1) {
+            atomicExch(&wc->next, 0);
+            atomicExch(&wc->done, 0);
+        }
+    }
+}
+
+// Rays given as xyz triples (hpt_cast) or as path state (render).
+// With COUNT, per-ray counters go to `counts` (HPT_CAST_COUNTS ints per ray)
+// when it is given, and their wave sums (nodes to the cap, leaves,
+// triangles) to the 64-bit `totals` when that is given.
+template <bool COUNT, bool PATHS>
+__global__ __launch_bounds__(kBlock) void k_cast(DevScene sc, const float *org3, const float *dir3,
+                                                 const float4 *org4, const float4 *dir4,
+                                                 const int *n_dev, int64_t n_host,
+                                                 WorkCounter *wc, int *out_id, float *out_t,
+                                                 int *counts, unsigned long long *totals) {
+    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+    const int lane = threadIdx.x & 63;
+    unsigned long long tot[3] = {0, 0, 0};
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&wc->next, 64);
+        base = __shfl(base, 0, 64);
+        if ((int64_t)base >= n) break;
+        const int64_t i = (int64_t)base + lane;
+        if (i < n) {
+            V3 sp, d;
+            if (PATHS) {
+                sp = xyz(org4[i]);
+                d = xyz(dir4[i]);
+            } else {
+                sp = mk(org3[3 * i], org3[3 * i + 1], org3[3 * i + 2]);
+                d = mk(dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]);
+            }
+            float t;
+            int c[HPT_CAST_COUNTS];
+            const int id = trace_ray<COUNT>(sc, sp, d, t, c);
+            out_id[i] = id;
+            out_t[i] = t;
+            if (COUNT) {
+                if (counts)
+                    for (int k = 0; k < HPT_CAST_COUNTS; ++k) counts[HPT_CAST_COUNTS * i + k] = c[k];
+                tot[0] += (unsigned)c[0];
+                tot[1] += (unsigned)c[1];
+                tot[2] += (unsigned)c[2];
+            }
+        }
+    }
+    if (COUNT && totals) {
+        for (int k = 0; k < 3; ++k) {
+            unsigned long long v = tot[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) atomicAdd(totals + k, v);
+        }
+    }
+    wave_done(wc);
+}
+
+// --------------------------------------------------------------- camera
+__global__ __launch_bounds__(kBlock) void k_camera(const ViewParams *views, int view_id,
+                                                   Paths out) {
+    const ViewParams &vp = views[view_id];
+    const int64_t n = (int64_t)vp.sx * vp.sy;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int gx = (int)(r % vp.sx), gy = (int)(r / vp.sx);
+        V3 d;
+        int64_t index;
+        if (vp.vr) {
+            const float lon = (float)(gx + vp.vr_shift) / (float)(vp.vr_sample_y * 2) * kPi * 2.f - kPi;
+            const float lat = (float)gy / (float)vp.vr_sample_y * kPi - kPi / 2.0f;
+            d = normalize3(mk(cosf(lat) * cosf(lon), sinf(lat), cosf(lat) * sinf(lon)));
+            index = (int64_t)vp.vr_sample_x * gy + gx;
+        } else {
+            const V3 up = mk(vp.up[0], vp.up[1], vp.up[2]);
+            const V3 right = mk(vp.right[0], vp.right[1], vp.right[2]);
+            V3 nd = cross(up, right);
+            nd = add(nd, scl((float)gy / (float)vp.sy * vp.height - vp.height * 0.5f, up));
+            nd = add(nd, scl((float)gx / (float)vp.sx * vp.width - vp.width * 0.5f, right));
+            d = normalize3(nd);
+            index = (int64_t)vp.sx * gy + gx;
+        }
+        const int64_t g = vp.ray_base + index;
+        out.org[g] = make_float4(vp.view[0], vp.view[1], vp.view[2], __int_as_float(view_id));
+        out.dir[g] = make_float4(d.x, d.y, d.z, 0.f);
+        out.thr[g] = make_float4(1.f, 1.f, 1.f, __int_as_float((int)g));
+    }
+}
+
+// ----------------------------------------------------- block compaction
+// Exclusive position of each surviving lane in the block's output run and
+// the run's base in the destination, from wave64 ballots and an LDS scan of
+// the per-wave totals; one atomic per block.
+__device__ __forceinline__ int64_t block_compact(bool alive, int *counter, int &pos_in_run) {
+    __shared__ int wave_tot[kBlock / 64];
+    __shared__ int run_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t mask = __ballot(alive);
+    const int before = __popcll(mask & ((1ULL << lane) - 1ULL));
+    if (lane == 0) wave_tot[wave] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            const int c = wave_tot[w];
+            wave_tot[w] = acc;
+            acc += c;
+        }
+        run_base = acc ? atomicAdd(counter, acc) : 0;
+    }
+    __syncthreads();
+    pos_in_run = wave_tot[wave] + before;
+    const int64_t base = run_base;
+    __syncthreads();  // LDS reused by the next call
+    return base;
+}
+
+// -------------------------------------------------------------- primary
+// After the primary cast of one view: misses write background * spp once
+// (kernel.cl:193-207 with sample_n = spp), hits are cached and listed.
+__global__ __launch_bounds__(kBlock) void k_primary(const ViewParams *views, int view_id,
+                                                    const int *hit_id, const float *hit_t,
+                                                    float *accum, int2 *prim_hit,
+                                                    int *prim_list, int *prim_count) {
+    const ViewParams &vp = views[view_id];
+    const int64_t n = (int64_t)vp.sx * vp.sy;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int64_t r = base + threadIdx.x;
+        bool alive = false;
+        int64_t g = 0;
+        if (r < n) {
+            g = vp.ray_base + r;
+            const int id = hit_id[g];
+            float *acc = accum + 3 * g;
+            if (id < 0) {
+                const float s = (float)vp.sample;
+                acc[0] = s * (vp.background[0] * 1.f);
+                acc[1] = s * (vp.background[1] * 1.f);
+                acc[2] = s * (vp.background[2] * 1.f);
+            } else {
+                acc[0] = acc[1] = acc[2] = 0.f;
+                prim_hit[g] = make_int2(id, __float_as_int(hit_t[g]));
+                alive = true;
+            }
+        }
+        int pos;
+        const int64_t run = block_compact(alive, prim_count + view_id, pos);
+        if (alive) prim_list[vp.ray_base + run + pos] = (int)g;
+    }
+}
+
+// --------------------------------------------------------------- expand
+// Restore the cached primary hits of every view taking part in a pass
+// into the path buffer (the reference copies data_initial_mem and the S1
+// cache, trace_runner.cpp:223-229), seeding each path's own LCG stream.
+__global__ __launch_bounds__(kBlock) void k_expand(const ViewParams *views, const Segment *segs,
+                                                   int n_segs, int64_t total, const int *prim_list,
+                                                   const int2 *prim_hit, const float4 *prim_dir,
+                                                   Paths out, int *out_id, float *out_t,
+                                                   int *live_count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *live_count = (int)total;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = n_segs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (segs[mid].out_offset <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const Segment &s = segs[lo];
+        const ViewParams &vp = views[s.view];
+        const int g = prim_list[s.prim_offset + (k - s.out_offset)];
+        const int2 h = prim_hit[g];
+        const float4 d = prim_dir[g];
+        out.org[k] = make_float4(vp.view[0], vp.view[1], vp.view[2], __int_as_float(s.view));
+        out.dir[k] = d;
+        out.thr[k] = make_float4(1.f, 1.f, 1.f, __int_as_float(g));
+        out.seed[k] = path_seed(vp.seed, (int64_t)g - vp.ray_base, s.pass);
+        out_id[k] = h.x;
+        out_t[k] = __int_as_float(h.y);
+    }
+}
+
+// ---------------------------------------------------------------- shade
+// read_imagef of s1_run with CLK_NORMALIZED_COORDS_TRUE | ADDRESS_REPEAT |
+// FILTER_LINEAR on a 512x512 RGBA UNORM8 layer (OpenCL 1.2, section 8.2).
+__device__ __forceinline__ V3 texel(const DevScene &sc, int layer, int i, int j) {
+    const uchar4 p = sc.textures[((int64_t)layer * 512 + j) * 512 + i];
+    return mk(p.x / 255.f, p.y / 255.f, p.z / 255.f);
+}
+
+__device__ __forceinline__ V3 read_texture(const DevScene &sc, int layer, float su, float sv) {
+    if (layer < 0) layer = 0;
+    if (layer > sc.n_textures - 1) layer = sc.n_textures - 1;
+    const float u = (su - floorf(su)) * 512.f;
+    const float v = (sv - floorf(sv)) * 512.f;
+    const float fu = floorf(u - 0.5f), fv = floorf(v - 0.5f);
+    int i0 = (int)fu, j0 = (int)fv;
+    int i1 = i0 + 1, j1 = j0 + 1;
+    if (i0 < 0) i0 = 512 + i0;
+    if (j0 < 0) j0 = 512 + j0;
+    if (i1 > 511) i1 = i1 - 512;
+    if (j1 > 511) j1 = j1 - 512;
+    if (i0 < 0 || i0 > 511 || i1 < 0 || i1 > 511) i0 = i1 = 0;
+    if (j0 < 0 || j0 > 511 || j1 < 0 || j1 > 511) j0 = j1 = 0;
+    const float a = (u - 0.5f) - fu, b = (v - 0.5f) - fv;
+    const V3 t00 = texel(sc, layer, i0, j0), t10 = texel(sc, layer, i1, j0);
+    const V3 t01 = texel(sc, layer, i0, j1), t11 = texel(sc, layer, i1, j1);
+    const float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
+    return mk(((w00 * t00.x + w10 * t10.x) + w01 * t01.x) + w11 * t11.x,
+              ((w00 * t00.y + w10 * t10.y) + w01 * t01.y) + w11 * t11.y,
+              ((w00 * t00.z + w10 * t10.z) + w01 * t01.z) + w11 * t11.z);
+}
+
+__device__ __forceinline__ V3 ld_point(const DevScene &sc, int i) { return xyz(sc.points[i]); }
+
+// One path: s1_run (kernel.cl:283-408) then, unless this is the last depth
+// level, the chosen s2 rule (kernel.cl:413-557).  Returns true when the
+// path continues; its next ray is left in o/d/strength/seed.
+__device__ __forceinline__ bool shade_path(const DevScene &sc, const ViewParams &vp, int id,
+                                           float t, V3 &o, V3 &d, V3 &strength, uint64_t &seed,
+                                           float *acc, bool last) {
+    const int4 g = sc.mesh[id];
+    const V3 A = ld_point(sc, g.x), B = ld_point(sc, g.y), Cc = ld_point(sc, g.z);
+    const hpt_material &mat = sc.materials[g.w];
+    const V3 P = add(o, scl(t, d));
+    const V3 f1 = sub(A, P), f2 = sub(B, P), f3 = sub(Cc, P);
+    const float area_all = length3(cross(sub(B, A), sub(Cc, A)));
+    const float a1 = length3(cross(f2, f3)) / area_all;
+    const float a2 = length3(cross(f1, f3)) / area_all;
+    const float a3 = length3(cross(f1, f2)) / area_all;
+    V3 normal;
+    if (sc.n_normals == 0) {
+        normal = normalize3(cross(sub(B, A), sub(Cc, A)));
+    } else {
+        normal = add(add(scl(a1, xyz(sc.normals[g.x])), scl(a2, xyz(sc.normals[g.y]))),
+                     scl(a3, xyz(sc.normals[g.z])));
+    }
+    float uvx = 0.f, uvy = 0.f;
+    const bool textured = mat.ambient_texture_id >= 0 || mat.diffuse_texture_id >= 0;
+    if (textured) {
+        if (sc.n_texcoords == 0) {
+            const float l = sqrtf(P.x * P.x + P.y * P.y);
+            uvx = P.x / l;
+            uvy = P.y / l;
+        } else {
+            const float2 ta = sc.texcoords[g.x], tb = sc.texcoords[g.y], tc = sc.texcoords[g.z];
+            uvx = (a1 * ta.x + a2 * tb.x) + a3 * tc.x;
+            uvy = (a1 * ta.y + a2 * tb.y) + a3 * tc.y;
+        }
+        uvy = 1.f - uvy;
+    }
+    V3 ambient = mk(mat.ambient[0], mat.ambient[1], mat.ambient[2]);
+    if (mat.ambient_texture_id >= 0) ambient = read_texture(sc, mat.ambient_texture_id, uvx, uvy);
+    V3 result = mulv(strength, add(ambient, mk(mat.emission[0], mat.emission[1], mat.emission[2])));
+    result = scl(fabsf(dot(normal, d)), result);
+    acc[0] += result.x;
+    acc[1] += result.y;
+    acc[2] += result.z;
+
+    if (last || length3(strength) < kGeneralThreshold) return false;
+    const float rn = lcg_randf(seed);
+    int branch;
+    if (rn < mat.specular_possibility) {
+        strength = mulv(strength, mk(mat.specular[0], mat.specular[1], mat.specular[2]));
+        branch = 1;
+    } else if (rn < mat.refract_possibility) {
+        strength = scl(1.0f - mat.dissolve, strength);
+        branch = 0;
+    } else {
+        V3 diffuse = mk(mat.diffuse[0], mat.diffuse[1], mat.diffuse[2]);
+        if (mat.diffuse_texture_id >= 0) diffuse = read_texture(sc, mat.diffuse_texture_id, uvx, uvy);
+        strength = mulv(strength, diffuse);
+        branch = (rn < mat.diffuse_possibility && vp.disable_diffuse == 0) ? 2 : 3;
+    }
+
+    if (branch == 0) {  // s2_refract_run
+        const float cos_alpha = dot(d, neg(normal));
+        float reverse = 1.0f;
+        if (cos_alpha < 0) reverse = -1;
+        const float alpha = acosf(cos_alpha);
+        const V3 p = scl(cos_alpha, normal);
+        const V3 q = normalize3(add(d, p));
+        const float sin_beta = sinf(alpha) * powf(mat.optical_density, -reverse);
+        V3 fd;
+        if (sin_beta <= 1.f) {
+            const float beta = asinf(sin_beta);
+            fd = add(scl(-reverse * cosf(beta), normal), scl(sin_beta, q));
+        } else {
+            fd = add(add(d, p), p);
+        }
+        o = add(P, scl(kStepEpsilon, fd));
+        d = fd;
+        return true;
+    }
+    if (branch == 1) {  // s2_specular_run
+        V3 rv = lcg_randf3(seed);
+        const float sh = mat.shininess;
+        rv = mk(rv.x / 2.0f / sh, rv.y / 2.0f / sh, rv.z / 2.0f / sh);
+        const float dd = dot(d, normal);
+        const V3 projection = scl(dd, normal);
+        V3 refl = sub(d, scl(2.0f, projection));
+        refl = add(refl, rv);
+        o = add(P, scl(kStepEpsilon, refl));
+        d = refl;
+        return true;
+    }
+    if (branch == 2) {  // s2_diffuse_run
+        const bool front = dot(d, normal) < 0;
+        V3 p = lcg_randf3(seed);
+        float dn = dot(p, normal);
+        if (dn < 0) {
+            dn = -dn;
+            if (front) p = neg(p);
+        }
+        strength = scl(dn, strength);
+        o = add(P, scl(kStepEpsilon, p));
+        d = p;
+        return true;
+    }
+    // s2_light_run
+    if (sc.n_lights == 0) return false;
+    const bool front = dot(d, normal) < 0;
+    const int li = (int)(lcg_rand(seed) % (uint32_t)sc.n_lights);
+    const int4 l = sc.lights[li];
+    const V3 pa = ld_point(sc, l.x), pb = ld_point(sc, l.y), pc = ld_point(sc, l.z);
+    float rx = lcg_randf(seed), ry = lcg_randf(seed);
+    if (rx + ry > 1) {
+        rx = 1 - rx;
+        ry = 1 - ry;
+    }
+    const V3 point = add(add(pa, scl(rx, sub(pb, pa))), scl(ry, sub(pc, pa)));
+    const V3 p = normalize3(sub(point, P));
+    const float dl = dot(p, normal);
+    if ((dl > 0) == front) {
+        strength = scl(fabsf(dl), strength);
+        o = add(P, scl(kStepEpsilon, p));
+        d = p;
+        return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, const ViewParams *views,
+                                                  int depth_index, const int *n_in, Paths in,
+                                                  const int *hit_id, const float *hit_t,
+                                                  Paths out, int *n_out, float *accum) {
+    const int64_t n = *n_in;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int64_t k = base + threadIdx.x;
+        bool alive = false;
+        float4 o4 = make_float4(0, 0, 0, 0), d4 = o4, th = o4;
+        uint64_t seed = 0;
+        V3 o, d, strength;
+        if (k < n) {
+            o4 = in.org[k];
+            d4 = in.dir[k];
+            th = in.thr[k];
+            seed = in.seed[k];
+            const int view = __float_as_int(o4.w);
+            const int gpix = __float_as_int(th.w);
+            const ViewParams &vp = views[view];
+            float *acc = accum + 3 * (int64_t)gpix;
+            o = xyz(o4);
+            d = xyz(d4);
+            strength = xyz(th);
+            const int id = hit_id[k];
+            if (id < 0) {
+                // Miss: background * strength * sample_n with sample_n = 1.
+                acc[0] += 1.f * (vp.background[0] * strength.x);
+                acc[1] += 1.f * (vp.background[1] * strength.y);
+                acc[2] += 1.f * (vp.background[2] * strength.z);
+            } else {
+                alive = shade_path(sc, vp, id, hit_t[k], o, d, strength, seed, acc,
+                                   depth_index >= vp.depth - 1);
+            }
+        }
+        int pos;
+        const int64_t run = block_compact(alive, n_out, pos);
+        if (alive) {
+            const int64_t w = run + pos;
+            out.org[w] = make_float4(o.x, o.y, o.z, o4.w);
+            out.dir[w] = make_float4(d.x, d.y, d.z, 0.f);
+            out.thr[w] = make_float4(strength.x, strength.y, strength.z, th.w);
+            out.seed[w] = seed;
+        }
+    }
+}
+
+// -------------------------------------------------------------- resolve
+__device__ __forceinline__ unsigned char unorm8(float f) {
+    const float x = f * 255.0f;
+    if (!(x > 0.f)) return 0;  // also NaN
+    if (x >= 255.f) return 255;
+    return (unsigned char)rintf(x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_resolve(const ViewParams *views, int view_id,
+                                                    int width, int height, int ssx, int ssy,
+                                                    float brightness, const float *accum,
+                                                    uchar4 *rgba) {
+    const ViewParams &vp = views[view_id];
+    const int64_t n = (int64_t)width * height;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int gx = (int)(p % width), gy = (int)(p / width);
+        float px = 0.f, py = 0.f, pz = 0.f;
+        for (int i = 0; i < ssx; ++i)
+            for (int j = 0; j < ssy; ++j) {
+                const int64_t row = (int64_t)gy * ssy + j, col = (int64_t)gx * ssx + i;
+                const int64_t index = vp.ray_base + row * width * ssx + col;
+                px += accum[index * 3];
+                py += accum[index * 3 + 1];
+                pz += accum[index * 3 + 2];
+            }
+        const float nss = (float)(ssx * ssy), sp = (float)vp.sample;
+        px /= nss; py /= nss; pz /= nss;
+        px /= sp; py /= sp; pz /= sp;
+        px *= brightness; py *= brightness; pz *= brightness;
+        px = 1.f - expf(-px);
+        py = 1.f - expf(-py);
+        pz = 1.f - expf(-pz);
+        rgba[p] = make_uchar4(unorm8(px), unorm8(py), unorm8(pz), 0);
+    }
+}
+
+// ------------------------------------------------------------ launchers
+static int grid_for(int64_t n, int cap) {
+    int64_t g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const float4 *org4,
+                 const float4 *dir4, const int *n_dev, int64_t n_host, WorkCounter *wc,
+                 int *out_id, float *out_t, int *counts, unsigned long long *totals, int grid,
+                 hipStream_t st) {
+    const bool paths = org4 != nullptr;
+    if (counts || totals) {
+        if (paths)
+            k_cast<true, true><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, out_t, counts, totals);
+        else
+            k_cast<true, false><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, out_t, counts, totals);
+    } else {
+        if (paths)
+            k_cast<false, true><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, out_t, counts, totals);
+        else
+            k_cast<false, false><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, out_t, counts, totals);
+    }
+}
+
+void launch_camera(const ViewParams *views, int view_id, int64_t n, Paths out, hipStream_t st) {
+    k_camera<<<grid_for(n, 8192), kBlock, 0, st>>>(views, view_id, out);
+}
+
+void launch_primary(const ViewParams *views, int view_id, int64_t n, const int *hit_id,
+                    const float *hit_t, float *accum, int2 *prim_hit, int *prim_list,
+                    int *prim_count, hipStream_t st) {
+    k_primary<<<grid_for(n, 8192), kBlock, 0, st>>>(views, view_id, hit_id, hit_t, accum,
+                                                     prim_hit, prim_list, prim_count);
+}
+
+void launch_expand(const ViewParams *views, const Segment *segs, int n_segs, int64_t total,
+                   const int *prim_list, const int2 *prim_hit, const float4 *prim_dir, Paths out,
+                   int *out_id, float *out_t, int *live_count, hipStream_t st) {
+    k_expand<<<grid_for(total, 8192), kBlock, 0, st>>>(views, segs, n_segs, total, prim_list,
+                                                        prim_hit, prim_dir, out, out_id, out_t,
+                                                        live_count);
+}
+
+void launch_shade(const DevScene &sc, const ViewParams *views, int depth_index, const int *n_in,
+                  Paths in, const int *hit_id, const float *hit_t, Paths out, int *n_out,
+                  float *accum, int grid, hipStream_t st) {
+    k_shade<<<grid, kBlock, 0, st>>>(sc, views, depth_index, n_in, in, hit_id, hit_t, out, n_out,
+                                     accum);
+}
+
+void launch_resolve(const ViewParams *views, int view_id, int width, int height, int ssx, int ssy,
+                    float brightness, const float *accum, uchar4 *rgba, hipStream_t st) {
+    k_resolve<<<grid_for((int64_t)width * height, 8192), kBlock, 0, st>>>(
+        views, view_id, width, height, ssx, ssy, brightness, accum, rgba);
+}
+
+int occupancy_blocks_cast() {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cast<false, true>, kBlock, 0) != hipSuccess) b = 4;
+    return b;
+}
+
+int occupancy_blocks_shade() {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shade, kBlock, 0) != hipSuccess) b = 4;
+    return b;
+}
+
+}  // namespace hpt

@@ -1,0 +1,87 @@
+// Device data layouts and kernel declarations of the wavefront path tracer.
+//
+// Scene in HBM (built once per scene by hpt_scene_create):
+//   nodes  : 32 bytes per KD node, pre-order.  float4 {box_start, a},
+//            float4 {box_end, b}; a/b are int bits.  Interior: a = index of
+//            the first node after the subtree (where the reference's
+//            sibling/parent walk continues, kernel.cl:153-167), b = -1, and
+//            the left child is the next node.  Leaf: a = triangle count,
+//            b = offset of its first triangle record (count 0 = empty leaf).
+//   tris   : 48 bytes per leaf entry, in leaf-list order: float4 {pa, id},
+//            float4 {e1 = pb - pa, 0}, float4 {e2 = pc - pa, 0}.
+//   shading arrays keep the reference's layouts (float4 points/normals,
+//   float2 texcoords, int4 mesh and lights, 96-byte materials, RGBA8 texels).
+//
+// Ray state is structure-of-arrays, one entry per live path:
+//   org  float4 {x, y, z, view id bits}
+//   dir  float4 {x, y, z, 0}
+//   thr  float4 {strength xyz, global pixel-sample index bits}
+//   seed uint64 (48-bit LCG state)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hptracing.h"
+
+namespace hpt {
+
+struct DevScene {
+    const float4 *nodes;
+    int32_t n_nodes;
+    const float4 *tris;
+    const float4 *points;
+    const float4 *normals;
+    int32_t n_normals;
+    const float2 *texcoords;
+    int32_t n_texcoords;
+    const int4 *mesh;
+    const hpt_material *materials;
+    const int4 *lights;
+    int32_t n_lights;
+    const uchar4 *textures;
+    int32_t n_textures;
+};
+
+struct Paths {
+    float4 *org;
+    float4 *dir;
+    float4 *thr;
+    uint64_t *seed;
+};
+
+// Per-view constants of one render request inside a launch.
+struct ViewParams {
+    float view[3];
+    float background[3];
+    int32_t disable_diffuse;
+    int32_t depth;
+    int32_t sample;
+    int32_t sx, sy;              // camera rays per row / column (W*ssx, H*ssy)
+    int32_t vr;                  // equirectangular camera
+    int32_t vr_shift, vr_sample_x, vr_sample_y;
+    float up[3], right[3];
+    float width, height;
+    uint64_t seed;
+    int64_t ray_base;            // first global pixel-sample index of the view
+};
+
+// A run of primary hits of one view expanded into the path buffer.
+struct Segment {
+    int32_t view;
+    int32_t pass;
+    int64_t prim_offset;         // into the primary-hit list
+    int64_t count;
+    int64_t out_offset;          // prefix sum of counts
+};
+
+// Work-distribution counters of a persistent kernel: `next` hands out ray
+// chunks, `done` counts finished waves; the last wave re-arms both.
+struct WorkCounter {
+    int32_t next;
+    int32_t done;
+};
+
+constexpr int kBlock = 256;
+
+}  // namespace hpt

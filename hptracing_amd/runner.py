@@ -1,0 +1,111 @@
+"""Python face of the GPU renderer: ``TraceRunner`` mirrors hp::TraceRunner
+(src/hp/trace_runner.h:21-66) over the C ABI of libhptracing.so.
+
+``TraceRunner(scene)`` uploads a ``KDTree`` to HBM once (the reference's
+constructor, trace_runner.cpp:41-108); ``run(...)`` takes the argument list
+of ``TraceRunner::run`` (trace_runner.h:53-65) and returns the RGBA8 image
+that ``get_image`` produces.  ``cast`` exposes the ray-cast stage alone.
+Errors raise ``HptError`` (the reference throws cl::Error).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .scene import KDTree
+
+
+def _f3(v: Sequence[float]):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def make_view(view_p, top_dir, right_dir, width, height, sample_x, sample_y,
+              supersample_x=1, supersample_y=1, background_color=(0, 0, 0), sample=10,
+              depth=6, disable_diffuse=0, brightness=1.0, seed=1) -> N.View:
+    v = N.View()
+    v.view, v.up, v.right = _f3(view_p), _f3(top_dir), _f3(right_dir)
+    v.width, v.height = float(width), float(height)
+    v.sample_x, v.sample_y = int(sample_x), int(sample_y)
+    v.supersample_x, v.supersample_y = int(supersample_x), int(supersample_y)
+    v.background = _f3(background_color)
+    v.sample, v.depth, v.disable_diffuse = int(sample), int(depth), int(disable_diffuse)
+    v.brightness = float(brightness)
+    v.seed = int(seed) & ((1 << 64) - 1)
+    return v
+
+
+class TraceRunner:
+    """Device-resident scene plus render / cast entry points."""
+
+    def __init__(self, scene: KDTree):
+        self.scene = scene
+        self._h = C.c_void_p()
+        d = scene.desc()
+        N.check_device(N.device().hpt_scene_create(C.byref(d), C.byref(self._h)))
+        st = N.SceneStats()
+        N.check_device(N.device().hpt_scene_stats_get(self._h, C.byref(st)))
+        self.stats = {k: getattr(st, k) for k, _ in N.SceneStats._fields_}
+        self.last_stats: Optional[N.RenderStats] = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def run(self, view_p, top_dir, right_dir, width, height, sample_x, sample_y,
+            supersample_x=1, supersample_y=1, background_color=(0, 0, 0), sample=10,
+            depth=6, disable_diffuse=0, brightness=1.0, seed=1, return_accum=False):
+        v = make_view(view_p, top_dir, right_dir, width, height, sample_x, sample_y,
+                      supersample_x, supersample_y, background_color, sample, depth,
+                      disable_diffuse, brightness, seed)
+        return self.render(v, return_accum=return_accum)
+
+    def render(self, v: N.View, return_accum=False, time_kernels=False, count_work=False):
+        rgba = np.zeros((v.sample_y, v.sample_x, 4), np.uint8)
+        R = v.sample_x * v.supersample_x * v.sample_y * v.supersample_y
+        accum = np.zeros((R, 3), np.float32) if return_accum else None
+        stats = N.RenderStats()
+        flags = (N.HPT_RENDER_TIME_KERNELS if time_kernels else 0) | \
+                (N.HPT_RENDER_COUNT_WORK if count_work else 0)
+        N.check_device(N.device().hpt_render_ex(
+            self._h, C.byref(v), rgba.ctypes.data, accum.ctypes.data if return_accum else None,
+            C.byref(stats), flags))
+        self.last_stats = stats
+        return (rgba, accum) if return_accum else rgba
+
+    def cast(self, org, dir, counted=False):
+        """Cast rays given as (n,3) float32 arrays (numpy or torch, any device);
+        returns (tri_id int32, t float32[, counts (n,4) int32]) as torch CUDA
+        tensors.  Runs on torch's current stream."""
+        import torch
+        dev = torch.device("cuda")
+        org_t = torch.as_tensor(org, dtype=torch.float32).to(dev).contiguous()
+        dir_t = torch.as_tensor(dir, dtype=torch.float32).to(dev).contiguous()
+        n = org_t.shape[0]
+        if org_t.shape != (n, 3) or dir_t.shape != (n, 3):
+            raise ValueError("org and dir must be (n, 3)")
+        ids = torch.empty(n, dtype=torch.int32, device=dev)
+        t = torch.empty(n, dtype=torch.float32, device=dev)
+        stream = torch.cuda.current_stream().cuda_stream
+        if counted:
+            counts = torch.empty((n, N.HPT_CAST_COUNTS), dtype=torch.int32, device=dev)
+            N.check_device(N.device().hpt_cast_counted(
+                self._h, org_t.data_ptr(), dir_t.data_ptr(), n, ids.data_ptr(), t.data_ptr(),
+                counts.data_ptr(), stream))
+            return ids, t, counts
+        N.check_device(N.device().hpt_cast(
+            self._h, org_t.data_ptr(), dir_t.data_ptr(), n, ids.data_ptr(), t.data_ptr(), stream))
+        return ids, t
+
+    def close(self):
+        if self._h:
+            N.device().hpt_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

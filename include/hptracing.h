@@ -1,0 +1,222 @@
+/*
+ * hptracing.h - C ABI of the MI355X wavefront path tracer.
+ *
+ * This header is the drop-in boundary that replaces the OpenCL host layer of
+ * hpTracing (`hp::TraceRunner`, src/hp/trace_runner.h:21-66) and the OpenCL
+ * kernel argument lists of src/hp/cl_src/kernel.cl.  Everything here is plain
+ * C: opaque handles, pointers and sizes, `int` status codes and a
+ * thread-local error string.  No HIP or torch type appears in a signature;
+ * a HIP stream is passed as `void *` (NULL = the library's own stream).
+ *
+ * Two shared libraries implement it:
+ *   libhpt_host.so  - CPU side: OBJ/MTL loading, light table, textures,
+ *                     KD-tree build and flatten   (hpt_host_*)
+ *   libhptracing.so - GPU side: HIP kernels for gfx950 (hpt_scene_*, hpt_cast,
+ *                     hpt_render, hpt_batch_render, hpt_server_*)
+ */
+#ifndef HPTRACING_H
+#define HPTRACING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+    HPT_OK = 0,
+    HPT_ERR_INVALID = -1, /* bad argument (null pointer, size out of range)  */
+    HPT_ERR_IO = -2,      /* file missing or unreadable                       */
+    HPT_ERR_SCENE = -3,   /* malformed scene (bad index, missing material)    */
+    HPT_ERR_HIP = -4,     /* HIP runtime error                                */
+    HPT_ERR_NOMEM = -5,   /* host or device allocation failed                 */
+    HPT_ERR_BUDGET = -6,  /* request larger than the device-memory budget     */
+    HPT_ERR_STATE = -7,   /* call not valid in the object's current state     */
+    HPT_ERR_TIMEOUT = -8  /* wait ended before the request completed         */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *hpt_last_error(void);
+/* Same for the CPU-side library (hpt_host_* calls). */
+const char *hpt_host_last_error(void);
+
+/* ------------------------------------------------------------ scene types */
+/* Material: same fields, order and 96-byte layout as the reference's
+ * `Material` (src/hp/cl_src/types.h:277-292; cl_float3 occupies 16 bytes). */
+typedef struct {
+    float emission[4];
+    float ambient[4];
+    float diffuse[4];
+    float specular[4];
+    float optical_density;
+    float dissolve;
+    float shininess;
+    int32_t diffuse_texture_id;
+    int32_t ambient_texture_id;
+    float specular_possibility;
+    float refract_possibility;
+    float diffuse_possibility;
+} hpt_material;
+
+/* KD node exactly as `KDTree::getData()` emits it (KDTreeNodeHeader,
+ * src/hp/cl_src/types.h:294-301, 48 bytes). */
+typedef struct {
+    float box_start[4];
+    float box_end[4];
+    int32_t child;   /* first (left) child, -1 for a leaf                    */
+    int32_t parent;  /* -1 for the root                                      */
+    int32_t sibling; /* right sibling, set on left children only, else -1    */
+    int32_t data;    /* offset of [count, tri ids...] in leaf_data, or -1    */
+} hpt_kd_node;
+
+/* Texture layers are 512 x 512 RGBA8 (Scene::texture_width/height,
+ * src/hp/scene/base.cpp:34-35). */
+#define HPT_TEXTURE_SIZE 512
+/* Entries of the emitter table (LIGHTS_TOTAL_NUMBER, scene/base.cpp:228). */
+#define HPT_LIGHT_TABLE_SIZE 256
+
+/* Flat scene arrays, i.e. everything `TraceRunner::TraceRunner` uploads
+ * (src/hp/trace_runner.cpp:134-155).  Points and normals use the cl_float3
+ * layout of the reference host vectors (4 floats, the 4th ignored);
+ * texcoords are 2 floats; mesh and lights are int4 (v0, v1, v2, material). */
+typedef struct {
+    const float *points;       int64_t n_points;
+    const float *normals;      int64_t n_normals;
+    const float *texcoords;    int64_t n_texcoords;
+    const int32_t *mesh;       int64_t n_triangles;
+    const hpt_material *materials; int64_t n_materials;
+    const int32_t *lights;     int64_t n_lights;    /* 0 or 256 entries */
+    const hpt_kd_node *nodes;  int64_t n_nodes;
+    const int32_t *leaf_data;  int64_t n_leaf_data;
+    const uint8_t *textures;   int64_t n_textures;  /* layers of 512*512*4 */
+} hpt_scene_desc;
+
+/* ======================================================= CPU side (host) */
+typedef struct hpt_host_scene hpt_host_scene;
+
+/* Load an OBJ (+ its mtllib) and build the KD tree: the reference's
+ * `KDTree(filename, mtl_basepath)` (src/hp/scene/kdtree.cpp:235-246) which
+ * runs `Scene::Scene` (src/hp/scene/base.cpp:37-158).  `mtl_basepath` may be
+ * NULL or "" (then the mtllib name is used as given, tiny_obj_loader.cpp:
+ * 485-489).  `n_threads` <= 0 picks the hardware concurrency; the tree does
+ * not depend on it. */
+int hpt_host_load_obj(const char *obj_path, const char *mtl_basepath,
+                      int n_threads, hpt_host_scene **out);
+
+/* Build a scene from in-memory arrays instead of an OBJ file: points are
+ * xyz triples, mesh is int4 (v0, v1, v2, material).  Runs the same light
+ * table (scene/base.cpp:160-198) and KD build as hpt_host_load_obj. */
+int hpt_host_from_arrays(const float *points_xyz, int64_t n_points,
+                         const int32_t *mesh, int64_t n_triangles,
+                         const hpt_material *materials, int64_t n_materials,
+                         int n_threads, hpt_host_scene **out);
+
+/* Fill `out` with pointers into the scene (valid until hpt_host_free):
+ * the flattened tree of `KDTree::getData()` (kdtree.cpp:248-292) plus the
+ * Scene vectors. */
+int hpt_host_describe(const hpt_host_scene *scene, hpt_scene_desc *out);
+
+/* Material probabilities as Scene::Scene computes them (base.cpp:85-95). */
+int hpt_host_material_from_mtl(const float ka[3], const float kd[3],
+                               const float ks[3], const float ke[3],
+                               float ior, float dissolve, float shininess,
+                               hpt_material *out);
+
+void hpt_host_free(hpt_host_scene *scene);
+
+/* Seconds spent in the KD build of the last load on this scene. */
+double hpt_host_build_seconds(const hpt_host_scene *scene);
+
+/* ====================================================== GPU side (device) */
+typedef struct hpt_scene hpt_scene;
+
+/* Device-resident scene: replaces the uploads of `TraceRunner::TraceRunner`
+ * (trace_runner.cpp:41-108).  Copies and repacks every array into HBM; the
+ * descriptor may be freed afterwards. */
+int hpt_scene_create(const hpt_scene_desc *desc, hpt_scene **out);
+int hpt_scene_destroy(hpt_scene *scene);
+
+typedef struct {
+    int64_t n_nodes, n_leaf_entries, n_triangles, n_textures;
+    int64_t device_bytes; /* HBM held by the scene */
+} hpt_scene_stats;
+int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
+
+/* Ray cast, the parity entry point of `kdtree_intersect`
+ * (kernel.cl:111-208): for each of n rays (origin, direction as xyz float
+ * triples in DEVICE memory) write the index of the hit triangle (-1 on a
+ * miss) and its distance t (-42 on a miss, the kernel's initial value).  The
+ * reference keeps only the triangle's int4 (kernel.cl:191); the index is the
+ * position of that int4 in the mesh.  Asynchronous on `stream`. */
+int hpt_cast(hpt_scene *scene, const float *org, const float *dir, int64_t n,
+             int32_t *tri_id, float *t, void *stream);
+
+/* Same, with traversal statistics written to `counts` (HPT_CAST_COUNTS
+ * int32 per ray, device memory): nodes box-tested up to the 512th non-empty
+ * leaf, non-empty leaves collected, triangles tested, box-hit non-empty
+ * leaves beyond the cap, nodes box-tested by the reference's full walk.
+ * Used to price the cast in algorithmic bytes. */
+#define HPT_CAST_COUNTS 5
+int hpt_cast_counted(hpt_scene *scene, const float *org, const float *dir,
+                     int64_t n, int32_t *tri_id, float *t, int32_t *counts,
+                     void *stream);
+
+/* One render request: the argument list of `TraceRunner::run`
+ * (trace_runner.h:53-65), plus an explicit RNG seed (the reference seeds
+ * from std::rand() after srand(time(0)), trace_runner.cpp:63,175-178).
+ * width/height are the angular extents (hptracing.cpp:74-75 passes
+ * W/H*angle and angle); height < 0 selects the equirectangular camera with
+ * shift = int(right[0]) (trace_runner.cpp:131-140). */
+typedef struct {
+    float view[3];
+    float up[3];
+    float right[3];
+    float width;
+    float height;
+    int32_t sample_x, sample_y;           /* output image size            */
+    int32_t supersample_x, supersample_y; /* 2 x 2 with the CLI's -x      */
+    float background[3];
+    int32_t sample;                       /* samples per pixel pass count */
+    int32_t depth;
+    int32_t disable_diffuse;
+    float brightness;
+    uint64_t seed;
+} hpt_view;
+
+#define HPT_MAX_DEPTH 64
+
+typedef struct {
+    int64_t primary_rays;               /* camera rays, cast once          */
+    int64_t rays_cast;                  /* all intersect work incl. primary */
+    int64_t live_per_depth[HPT_MAX_DEPTH]; /* rays cast at bounce i, summed over passes */
+    double seconds;                     /* wall time of the render call    */
+    /* With HPT_RENDER_TIME_KERNELS: summed device time (HIP events on the
+     * render stream) of every cast and every shade launch. */
+    double cast_ms, shade_ms;
+    int64_t cast_launches, shade_launches;
+    /* With HPT_RENDER_COUNT_WORK: traversal work summed over all casts
+     * (nodes box-tested up to the 512th leaf, non-empty leaves collected,
+     * triangles tested). */
+    int64_t node_visits, leaf_visits, tri_tests;
+} hpt_render_stats;
+
+#define HPT_RENDER_TIME_KERNELS 1u
+#define HPT_RENDER_COUNT_WORK 2u
+
+/* Render one view.  `rgba` (host, W*H*4 bytes, may be NULL) receives the
+ * tone-mapped image exactly as get_image writes it (kernel.cl:615-652;
+ * alpha 0, row 0 = camera row 0).  `accum` (host, W*ssx*H*ssy*3 floats,
+ * may be NULL) receives the raw per-sample radiance sums that the reference
+ * keeps in results_mem.  `stats` may be NULL. */
+int hpt_render(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
+               float *accum, hpt_render_stats *stats);
+/* hpt_render with HPT_RENDER_* flags. */
+int hpt_render_ex(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
+                  float *accum, hpt_render_stats *stats, unsigned flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPTRACING_H */
