@@ -269,7 +269,8 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
             const hpt_kd_node &nd = nodes[i];
             if (nd.child >= 0 && nd.child != i + 1)
                 fail(HPT_ERR_SCENE, "KD nodes are not in pre-order (child != index + 1)");
-            if (nd.child < 0 && esc[i] != i + 1)
+            // The walk must move forward at every step so that it ends.
+            if ((nd.child < 0 && esc[i] != i + 1) || esc[i] <= i || esc[i] > nn)
                 fail(HPT_ERR_SCENE, "KD sibling/parent links do not follow the pre-order");
             if (nd.child < 0 && nd.data >= 0) {
                 if (nd.data >= d->n_leaf_data) fail(HPT_ERR_SCENE, "leaf offset out of range");

@@ -1,0 +1,241 @@
+"""CPU side: OBJ/MTL loading, materials, textures, emitter table and the KD
+build, against the reference's known answers and closed forms."""
+from __future__ import annotations
+
+import os
+import textwrap
+
+import numpy as np
+import pytest
+
+from hptracing_amd import _native as N
+from hptracing_amd.scene import KDTree, material
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(textwrap.dedent(text))
+    return str(p)
+
+
+# ------------------------------------------------------------ known answers
+def test_light_table_known_answer(cornell):
+    """SceneCLTest.load (tests/src/test_scene_cl.cpp:19-27): 256 entries, 127
+    of them on the light quad's second fan triangle (vertex 14)."""
+    assert cornell.lights.shape == (256, 4)
+    assert int(np.sum(cornell.lights[:, 1] == 14)) == 127
+    light_tris = {tuple(t) for t in cornell.lights.tolist()}
+    assert light_tris == {(12, 13, 14, 5), (12, 14, 15, 5)}
+
+
+def test_cornell_mesh_layout(cornell):
+    # 18 faces in the file: 17 quads and... fan triangulation gives 34 triangles,
+    # per-shape vertex de-duplication gives 68 points.
+    assert cornell.geometries.shape == (34, 4)
+    assert cornell.points.shape == (68, 4)
+    assert cornell.geometries[0].tolist() == [0, 1, 2, 0]
+    assert cornell.geometries[1].tolist() == [0, 2, 3, 0]
+    assert cornell.n_textures == 0 and len(cornell.normals) == 0
+
+
+def test_material_probabilities(cornell):
+    """Scene::Scene (scene/base.cpp:85-95) on the fixture's materials."""
+    m = {i: cornell.materials[i] for i in range(len(cornell.materials))}
+    # white: Kd 1 1 1, d 1 -> diffuse only
+    f32 = np.float32
+    dl = np.sqrt(f32(1) + (f32(1) + f32(1)))
+    s = f32(np.float64(f32(f32(0) + dl) + f32(0)) + 1e-4)
+    assert m[0]["specular_possibility"] == 0
+    assert m[0]["diffuse_possibility"] == f32(f32(dl * f32(0.9)) / s)
+    # transparent: Ks .1, Ni 1.5, d 0
+    t = m[1]
+    assert t["optical_density"] == f32(1.5) and t["dissolve"] == 0
+    assert abs(t["specular_possibility"] - 0.1732 / 1.1733) < 1e-4
+    assert abs(t["refract_possibility"] - 1.0 / 1.1733 - t["specular_possibility"]) < 1e-4
+    # light: everything zero -> the light branch takes every draw
+    assert m[5]["emission"][:3].tolist() == [20, 20, 20]
+    assert m[5]["diffuse_possibility"] == 0
+
+
+def test_material_helper_matches_loader():
+    x = material(kd=(1, 1, 1))
+    assert abs(x["diffuse_possibility"] - 0.899948) < 1e-6
+
+
+# ------------------------------------------------------------- OBJ quirks
+def test_usemtl_drops_earlier_faces_and_fans(tmp_path):
+    write(tmp_path, "m.mtl", """
+        newmtl a
+        Kd 1 0 0
+        newmtl b
+        Kd 0 1 0
+        Ke 1 1 1
+    """)
+    p = write(tmp_path, "s.obj", """
+        mtllib m.mtl
+        v 0 0 0
+        v 1 0 0
+        v 1 1 0
+        v 0 1 0
+        v 0.5 1.5 0
+        o first
+        usemtl a
+        f 1 2 3
+        usemtl b
+        f 1 2 3 5 4
+        o second
+        usemtl a
+        f -5 -4 -3
+    """)
+    kd = KDTree(p, str(tmp_path) + "/")
+    g = kd.geometries
+    # the face before the second usemtl is dropped (tiny_obj_loader.cpp:629)
+    assert len(g) == 3 + 1
+    # fan of the pentagon: (1,2,3), (1,3,5), (1,5,4) with material b
+    assert g[:3, 3].tolist() == [1, 1, 1]
+    pts = kd.points[:, :3]
+    fan = [[tuple(pts[i]) for i in tri[:3]] for tri in g[:3]]
+    assert fan[0] == [(0, 0, 0), (1, 0, 0), (1, 1, 0)]
+    assert fan[1] == [(0, 0, 0), (1, 1, 0), (0.5, 1.5, 0)]
+    assert fan[2] == [(0, 0, 0), (0.5, 1.5, 0), (0, 1, 0)]
+    # per-shape vertex streams: the second shape re-emits its 3 vertices
+    assert len(kd.points) == 5 + 3
+    assert g[3].tolist() == [5, 6, 7, 0]
+    # material b emits: the light table points only at its triangles
+    assert set(map(tuple, kd.lights[:, :3].tolist())) <= {tuple(t[:3]) for t in g[:3].tolist()}
+
+
+def test_missing_material_and_file_errors(tmp_path):
+    p = write(tmp_path, "s.obj", """
+        v 0 0 0
+        v 1 0 0
+        v 0 1 0
+        f 1 2 3
+    """)
+    with pytest.raises(N.HptError) as e:
+        KDTree(p)
+    assert e.value.code == N.HPT_ERR_SCENE
+    with pytest.raises(N.HptError) as e:
+        KDTree(str(tmp_path / "missing.obj"))
+    assert e.value.code == N.HPT_ERR_IO
+
+
+def test_texture_pipeline(tmp_path):
+    """CImg normalize(0,255) + nearest resize to 512^2 + truncation
+    (scene/base.cpp:98-110, CImg.h:19093-19105, 21063-21138)."""
+    rng = np.random.default_rng(0)
+    src = rng.integers(20, 200, (3, 5, 3)).astype(np.uint8)  # h=3, w=5
+    with open(tmp_path / "t.ppm", "wb") as fh:
+        fh.write(b"P6\n5 3\n255\n" + src.tobytes())
+    write(tmp_path, "m.mtl", """
+        newmtl tex
+        Kd 1 1 1
+        map_Kd t.ppm
+    """)
+    p = write(tmp_path, "s.obj", """
+        mtllib m.mtl
+        v 0 0 0
+        v 1 0 0
+        v 0 1 0
+        vt 0 0
+        vt 1 0
+        vt 0 1
+        usemtl tex
+        f 1/1 2/2 3/3
+    """)
+    kd = KDTree(p, str(tmp_path) + "/")
+    assert kd.n_textures == 1
+    tex = kd.texture_data.reshape(512, 512, 4)
+    m, M = float(src.min()), float(src.max())
+    f32 = np.float32
+    norm = ((src.astype(f32) - f32(m)) / (f32(M) - f32(m)) * f32(255)).astype(f32)
+    ys = (np.arange(512) * 3) // 512
+    xs = (np.arange(512) * 5) // 512
+    expect = norm[ys][:, xs].astype(np.uint8)
+    assert np.array_equal(tex[:, :, :3], expect)
+    assert (tex[:, :, 3] == 0).all()
+    assert kd.materials[0]["diffuse_texture_id"] == 0
+
+
+# ------------------------------------------------------------ KD structure
+def _check_tree(kd):
+    nodes, leaf = kd.get_data()
+    n = len(nodes)
+    tris = kd.geometries
+    pts = kd.points[:, :3]
+    seen = np.zeros(len(tris), bool)
+    for i in range(n):
+        nd = nodes[i]
+        if nd["child"] >= 0:
+            assert nd["child"] == i + 1
+            assert nd["data"] == -1
+            left = nodes[i + 1]
+            right = left["sibling"]
+            assert right > i + 1 and nodes[right]["parent"] == i and left["parent"] == i
+            for c in (i + 1, right):
+                assert (nodes[c]["box_start"][:3] >= nd["box_start"][:3]).all()
+                assert (nodes[c]["box_end"][:3] <= nd["box_end"][:3]).all()
+        elif nd["data"] >= 0:
+            cnt = leaf[nd["data"]]
+            ids = leaf[nd["data"] + 1: nd["data"] + 1 + cnt]
+            seen[ids] = True
+            lo, hi = nd["box_start"][:3], nd["box_end"][:3]
+            tp = pts[tris[ids, :3]]
+            # every listed triangle reaches into the leaf box on every axis
+            assert (tp.max(axis=1) >= lo).all() and (tp.min(axis=1) <= hi).all()
+    assert seen.all()
+    root = nodes[0]
+    assert root["parent"] == -1 and root["sibling"] == -1
+    lo = pts[tris[:, :3]].reshape(-1, 3).min(0)
+    hi = pts[tris[:, :3]].reshape(-1, 3).max(0)
+    assert np.array_equal(root["box_start"][:3], (lo - np.float32(1e-3)).astype(np.float32))
+    assert np.array_equal(root["box_end"][:3], (hi + np.float32(1e-3)).astype(np.float32))
+
+
+def test_kdtree_invariants(cornell, room, soup_small):
+    for kd in (cornell, room, soup_small):
+        _check_tree(kd)
+
+
+def test_kdtree_matches_literal_restatement(cornell):
+    """The optimised builder (merged candidate lists, threads, freed arrays)
+    against a literal Python restatement of kdtree.cpp on the fixture."""
+    from kd_literal import build_literal
+    nodes, leaf = build_literal(cornell.points[:, :3], cornell.geometries)
+    got_nodes, got_leaf = cornell.get_data()
+    assert np.array_equal(leaf, got_leaf)
+    assert len(nodes) == len(got_nodes)
+    for a, b in zip(nodes, got_nodes):
+        assert np.array_equal(np.float32(a[0]), b["box_start"][:3])
+        assert np.array_equal(np.float32(a[1]), b["box_end"][:3])
+        assert (a[2], a[3], a[4], a[5]) == (b["child"], b["parent"], b["sibling"], b["data"])
+
+
+def test_kdtree_literal_soup():
+    from hptracing_amd import scenes
+    from kd_literal import build_literal
+    pts, mesh = scenes.soup(300, seed=9)
+    kd = KDTree.from_arrays(pts, mesh, np.array([material(kd=(0.5, 0.5, 0.5))]), n_threads=4)
+    nodes, leaf = build_literal(kd.points[:, :3], kd.geometries)
+    got_nodes, got_leaf = kd.get_data()
+    assert np.array_equal(leaf, got_leaf)
+    assert [tuple(x[2:]) for x in nodes] == [
+        (int(b["child"]), int(b["parent"]), int(b["sibling"]), int(b["data"])) for b in got_nodes]
+
+
+def test_kdtree_thread_count_independent():
+    from hptracing_amd import scenes
+    pts, mesh = scenes.soup(60000, seed=4)
+    mats = np.array([material(kd=(0.5, 0.5, 0.5))])
+    a = KDTree.from_arrays(pts, mesh, mats, n_threads=1)
+    b = KDTree.from_arrays(pts, mesh, mats, n_threads=8)
+    assert np.array_equal(a.nodes.view(np.uint8), b.nodes.view(np.uint8))
+    assert np.array_equal(a.leaf_data, b.leaf_data)
+
+
+def test_from_arrays_rejects_bad_indices():
+    pts = np.zeros((3, 3), np.float32)
+    mesh = np.array([[0, 1, 3, 0]], np.int32)
+    with pytest.raises(N.HptError) as e:
+        KDTree.from_arrays(pts, mesh, np.array([material()]))
+    assert e.value.code == N.HPT_ERR_SCENE
