@@ -1,0 +1,37 @@
+#!/bin/bash
+# Builds the reference's own device code for the GPU box -- TEST INFRASTRUCTURE.
+#
+#   oracle/_ref/ref_kernels.co    src/hp/cl_src/types.h.cl + kernel.cl, unmodified,
+#                                 compiled in place from /root/reference with ROCm's
+#                                 clang and its OpenCL device library for gfx950
+#   oracle/_ref/libref_harness.so oracle/ref_harness.hip: loads that code object and
+#                                 launches kdtree_intersect the way TraceRunner::run
+#                                 does (trace_runner.cpp:192-206)
+#
+# Build options: -cl-std=CL1.2 (the reference's sampler branch), -include types.h.cl
+# (the host prepends it, trace_runner.cpp:47-52), -cl-single-precision-constant
+# (double literals such as `/ 2.0` at kernel.cl:463 are float on a device
+# without fp64; clang rejects them otherwise), -cl-fp32-correctly-rounded-divide-sqrt
+# and -ffp-contract=off (IEEE division and no contraction, the arithmetic the
+# parity comparison is defined on).  Only the two outputs above are written;
+# no reference source is copied.
+set -euo pipefail
+REF=${HPT_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+CLANG=/opt/rocm/lib/llvm/bin/clang
+if [ ! -f "$REF/src/hp/cl_src/kernel.cl" ]; then
+    echo "build_ref: $REF not present, skipping" >&2
+    exit 0
+fi
+mkdir -p "$OUT"
+"$CLANG" -x cl -cl-std=CL1.2 -target amdgcn-amd-amdhsa -mcpu=gfx950 \
+    --rocm-path=/opt/rocm -Xclang -finclude-default-header \
+    -include "$REF/src/hp/cl_src/types.h.cl" \
+    -cl-single-precision-constant -cl-fp32-correctly-rounded-divide-sqrt -ffp-contract=off -O3 \
+    "$REF/src/hp/cl_src/kernel.cl" -o "$OUT/ref_kernels.co.tmp"
+mv "$OUT/ref_kernels.co.tmp" "$OUT/ref_kernels.co"
+/opt/rocm/bin/hipcc -std=c++17 -O2 -fPIC -shared "$HERE/ref_harness.hip" \
+    -o "$OUT/libref_harness.so.tmp" -Wno-unused-command-line-argument
+mv "$OUT/libref_harness.so.tmp" "$OUT/libref_harness.so"
+echo "build_ref: wrote $OUT/ref_kernels.co and $OUT/libref_harness.so" >&2

@@ -1,0 +1,80 @@
+"""hpt_cast against the reference's OWN kdtree_intersect (kernel.cl, compiled
+unmodified for gfx950 by oracle/build_ref.sh and launched by
+oracle/ref_harness.hip on identical ray batches).
+
+The bar of the ray-cast stage: identical hit triangles, t within 1e-6
+relative.  Skipped where the reference build is absent (it needs the
+reference sources at build time)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from hptracing_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+CO = os.path.join(REF_DIR, "ref_kernels.co")
+LIB = os.path.join(REF_DIR, "libref_harness.so")
+
+needs_ref = pytest.mark.skipif(not (os.path.exists(CO) and os.path.exists(LIB)),
+                               reason="reference kernels not built (oracle/build_ref.sh)")
+
+
+def ref_cast(kd, org, d):
+    lib = C.CDLL(LIB)
+    vp = C.c_void_p
+    lib.ref_kdtree_intersect.argtypes = [C.c_char_p, vp, C.c_int64, vp, C.c_int64, vp, C.c_int64,
+                                         vp, C.c_int64, vp, vp, C.c_int32, vp, vp]
+    lib.ref_last_error.restype = C.c_char_p
+    org = np.ascontiguousarray(org, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    n = len(org)
+    geo = np.empty((n, 4), np.int32)
+    t = np.empty(n, np.float32)
+    rc = lib.ref_kdtree_intersect(CO.encode(), kd.points.ctypes.data, len(kd.points),
+                                  kd.geometries.ctypes.data, len(kd.geometries),
+                                  kd.nodes.ctypes.data, len(kd.nodes), kd.leaf_data.ctypes.data,
+                                  len(kd.leaf_data), org.ctypes.data, d.ctypes.data, n,
+                                  geo.ctypes.data, t.ctypes.data)
+    assert rc == 0, lib.ref_last_error()
+    return geo, t
+
+
+def compare(kd, org, d):
+    import torch
+    from hptracing_amd.runner import TraceRunner
+    r = TraceRunner(kd)
+    ids, t = r.cast(org, d)
+    torch.cuda.synchronize()
+    ids, t = ids.cpu().numpy(), t.cpu().numpy()
+    ref_geo, ref_t = ref_cast(kd, org, d)
+    ours_geo = np.where(ids[:, None] >= 0, kd.geometries[np.maximum(ids, 0)], -1)
+    same = (ours_geo == ref_geo).all(axis=1)
+    hit = ids >= 0
+    rel = np.abs(t[hit & same] - ref_t[hit & same]) / np.maximum(np.abs(ref_t[hit & same]), 1e-30)
+    exact_t = np.mean(t[hit & same].view(np.int32) == ref_t[hit & same].view(np.int32))
+    print(f"\n[ref-opencl] rays {len(ids)} hits {hit.sum()} id-mismatch {(~same).sum()} "
+          f"max t rel {rel.max() if rel.size else 0:.3g} t bit-exact {exact_t:.6f}")
+    return same, rel
+
+
+@needs_ref
+def test_cast_matches_reference_kernel_room(room):
+    org, d = scenes.random_rays(100000, (0, 0, -2), (10, 10, 10), seed=31)
+    same, rel = compare(room, org, d)
+    assert same.all()
+    assert (rel <= 1e-6).all()
+
+
+@needs_ref
+def test_cast_matches_reference_kernel_soup(soup_small):
+    org, d = scenes.random_rays(100000, (0, 0, 0), (1, 1, 1), seed=32)
+    same, rel = compare(soup_small, org, d)
+    assert same.all()
+    assert (rel <= 1e-6).all()
