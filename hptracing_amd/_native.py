@@ -121,7 +121,23 @@ class RenderStats(C.Structure):
 
 HPT_RENDER_TIME_KERNELS = 1
 HPT_RENDER_COUNT_WORK = 2
+HPT_RENDER_NO_GRAPH = 4
 HPT_CAST_COUNTS = 5
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("device_budget_bytes", C.c_int64)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("queue_ms", C.c_double),
+                ("latency_ms", C.c_double), ("camera_rays", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int64), ("active", C.c_int64), ("completed", C.c_int64),
+                ("macro_passes", C.c_int64), ("rays_cast", C.c_int64),
+                ("busy_seconds", C.c_double), ("capacity_rays", C.c_int64)]
 
 
 _host = None
@@ -170,6 +186,13 @@ def device() -> C.CDLL:
         lib.hpt_render.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(RenderStats)]
         lib.hpt_render_ex.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(RenderStats),
                                       C.c_uint]
+        lib.hpt_batch_render.argtypes = [vp, C.POINTER(View), C.c_int, C.POINTER(vp),
+                                         C.POINTER(vp), C.POINTER(RenderStats), C.c_uint]
+        lib.hpt_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
+        lib.hpt_server_submit.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(C.c_int64)]
+        lib.hpt_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(RequestInfo)]
+        lib.hpt_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
+        lib.hpt_server_destroy.argtypes = [vp]
         _device = lib
     return _device
 

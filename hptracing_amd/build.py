@@ -55,9 +55,20 @@ def build_device() -> str:
     return out
 
 
+def build_cli() -> str:
+    """The hptracing command line, linked against both libraries."""
+    out = os.path.join(LIB_DIR, "hptracing")
+    src = os.path.join(PKG, "csrc", "cli", "hptracing.cpp")
+    _run(["g++", *HOST_FLAGS, f"-I{INCLUDE}", src, "-o", out + ".tmp", f"-L{LIB_DIR}",
+          "-lhpt_host", "-lhptracing", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> None:
     build_host()
     build_device()
+    build_cli()
 
 
 if __name__ == "__main__":

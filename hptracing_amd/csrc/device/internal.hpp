@@ -1,0 +1,185 @@
+// Internal state shared by the C ABI translation units of libhptracing.so:
+// error plumbing, device allocations, kernel timing and the scene object.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hptracing.h"
+#include "kernels.hpp"
+#include "launch.hpp"
+
+namespace hpt {
+
+inline thread_local std::string g_error;
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Fail{code, msg}; }
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) fail(HPT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        g_error.clear();
+        f();
+        return HPT_OK;
+    } catch (const Fail &e) {
+        g_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_error = "host allocation failed";
+        return HPT_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return HPT_ERR_INVALID;
+    }
+}
+
+// Device allocation owned by a scene or a workspace.
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    ~DevMem() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+            fail(HPT_ERR_NOMEM, "device allocation of " + std::to_string(n) + " bytes failed");
+        hip_check(e, "hipMalloc");
+        bytes = n;
+    }
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+template <class T>
+inline void upload(DevMem &m, const T *src, size_t count, int64_t &total) {
+    size_t n = sizeof(T) * std::max<size_t>(count, 1);
+    m.reserve(n);
+    if (count) hip_check(hipMemcpy(m.p, src, sizeof(T) * count, hipMemcpyHostToDevice), "upload");
+    total += (int64_t)n;
+}
+
+// Render buffers for up to `capacity` camera rays, grown on demand.
+struct Workspace {
+    int64_t capacity = 0;
+    DevMem prim_org, prim_dir, prim_thr, prim_id, prim_t, prim_hit, prim_list;
+    DevMem a_org, a_dir, a_thr, a_seed, b_org, b_dir, b_thr, b_seed;
+    DevMem hit_id, hit_t, accum, rgba, views, segs, live, prim_count, totals, pass, passes;
+    void ensure(int64_t rays, int64_t pixels, int n_passes, int depth, int n_views) {
+        if (rays > capacity) {
+            const size_t n = (size_t)rays;
+            prim_org.reserve(16 * n);
+            prim_dir.reserve(16 * n);
+            prim_thr.reserve(16 * n);
+            prim_id.reserve(4 * n);
+            prim_t.reserve(4 * n);
+            prim_hit.reserve(8 * n);
+            prim_list.reserve(4 * n);
+            a_org.reserve(16 * n);
+            a_dir.reserve(16 * n);
+            a_thr.reserve(16 * n);
+            a_seed.reserve(8 * n);
+            b_org.reserve(16 * n);
+            b_dir.reserve(16 * n);
+            b_thr.reserve(16 * n);
+            b_seed.reserve(8 * n);
+            hit_id.reserve(4 * n);
+            hit_t.reserve(4 * n);
+            accum.reserve(12 * n);
+            capacity = rays;
+        }
+        rgba.reserve(4 * (size_t)std::max<int64_t>(pixels, 1));
+        views.reserve(sizeof(hpt::ViewParams) * (size_t)std::max(n_views, 1));
+        segs.reserve(sizeof(hpt::Segment) * (size_t)std::max(n_passes * n_views, 1));
+        live.reserve(sizeof(int) * (size_t)n_passes * (depth + 1));
+        prim_count.reserve(sizeof(int) * (size_t)std::max(n_views, 1));
+        totals.reserve(4 * sizeof(unsigned long long));
+        pass.reserve(sizeof(int));
+        passes.reserve(sizeof(hpt::PassDesc) * (size_t)std::max(n_passes, 1));
+    }
+};
+
+constexpr int kCounterRing = 256;
+
+// Device time of a series of launches on one stream, from HIP event pairs
+// recorded around each launch (read after the stream is synchronized).
+struct KernelTimer {
+    bool on;
+    hipStream_t st;
+    std::vector<hipEvent_t> ev;
+    KernelTimer(bool enabled, hipStream_t s) : on(enabled), st(s) {}
+    ~KernelTimer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    void mark() {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        ev.push_back(e);
+        hip_check(hipEventRecord(e, st), "hipEventRecord");
+    }
+    void begin() {
+        if (on) mark();
+    }
+    void end() {
+        if (on) mark();
+    }
+    int64_t count() const { return (int64_t)ev.size() / 2; }
+    double total_ms() const {
+        double t = 0;
+        for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) t += ms;
+        }
+        return t;
+    }
+};
+
+}  // namespace hpt
+
+struct hpt_scene {
+    int device = 0;
+    hpt::DevScene dev{};
+    hpt::DevMem nodes, tris, points, normals, texcoords, mesh, materials, lights, textures,
+        counters;
+    int64_t device_bytes = 0;
+    int64_t n_nodes = 0, n_entries = 0, n_triangles = 0, n_textures = 0;
+    int cast_grid = 0, shade_grid = 0;
+    std::atomic<int> next_counter{0};
+    std::mutex mu;
+    hpt::Workspace ws;
+    hipStream_t stream = nullptr;
+
+    hpt::WorkCounter *counter() {
+        const int k = next_counter.fetch_add(1) % hpt::kCounterRing;
+        return counters.as<hpt::WorkCounter>() + k;
+    }
+};
+

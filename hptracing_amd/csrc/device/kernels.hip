@@ -97,10 +97,10 @@ __device__ __forceinline__ void wave_done(WorkCounter *wc) {
 template <bool COUNT, bool PATHS>
 __global__ __launch_bounds__(kBlock) void k_cast(DevScene sc, const float *org3, const float *dir3,
                                                  const float4 *org4, const float4 *dir4,
-                                                 const int *n_dev, int64_t n_host,
+                                                 CountRef n_dev, int64_t n_host,
                                                  WorkCounter *wc, int *out_id, float *out_t,
                                                  int *counts, unsigned long long *totals) {
-    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+    const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
     const int lane = threadIdx.x & 63;
     unsigned long long tot[3] = {0, 0, 0};
     for (;;) {
@@ -240,12 +240,17 @@ __global__ __launch_bounds__(kBlock) void k_primary(const ViewParams *views, int
 // Restore the cached primary hits of every view taking part in a pass
 // into the path buffer (the reference copies data_initial_mem and the S1
 // cache, trace_runner.cpp:223-229), seeding each path's own LCG stream.
-__global__ __launch_bounds__(kBlock) void k_expand(const ViewParams *views, const Segment *segs,
-                                                   int n_segs, int64_t total, const int *prim_list,
-                                                   const int2 *prim_hit, const float4 *prim_dir,
-                                                   Paths out, int *out_id, float *out_t,
-                                                   int *live_count) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *live_count = (int)total;
+// The pass comes from device memory (`pass`), its segments from `passes`.
+__global__ __launch_bounds__(kBlock) void k_expand(const ViewParams *views, const Segment *segs_all,
+                                                   const PassDesc *passes, const int *pass,
+                                                   const int *prim_list, const int2 *prim_hit,
+                                                   const float4 *prim_dir, Paths out, int *out_id,
+                                                   float *out_t, CountRef live) {
+    const PassDesc pd = passes[*pass];
+    const Segment *segs = segs_all + pd.seg_begin;
+    const int n_segs = pd.n_segs;
+    const int64_t total = pd.total;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *live.ptr() = (int)total;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
          k += (int64_t)gridDim.x * blockDim.x) {
         int lo = 0, hi = n_segs - 1;
@@ -429,10 +434,11 @@ __device__ __forceinline__ bool shade_path(const DevScene &sc, const ViewParams 
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, const ViewParams *views,
-                                                  int depth_index, const int *n_in, Paths in,
+                                                  int depth_index, CountRef n_in, Paths in,
                                                   const int *hit_id, const float *hit_t,
-                                                  Paths out, int *n_out, float *accum) {
-    const int64_t n = *n_in;
+                                                  Paths out, CountRef n_out_ref, float *accum) {
+    const int64_t n = *n_in.ptr();
+    int *n_out = n_out_ref.ptr();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
         const int64_t k = base + threadIdx.x;
@@ -521,7 +527,7 @@ static int grid_for(int64_t n, int cap) {
 }
 
 void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const float4 *org4,
-                 const float4 *dir4, const int *n_dev, int64_t n_host, WorkCounter *wc,
+                 const float4 *dir4, CountRef n_dev, int64_t n_host, WorkCounter *wc,
                  int *out_id, float *out_t, int *counts, unsigned long long *totals, int grid,
                  hipStream_t st) {
     const bool paths = org4 != nullptr;
@@ -549,20 +555,25 @@ void launch_primary(const ViewParams *views, int view_id, int64_t n, const int *
                                                      prim_hit, prim_list, prim_count);
 }
 
-void launch_expand(const ViewParams *views, const Segment *segs, int n_segs, int64_t total,
-                   const int *prim_list, const int2 *prim_hit, const float4 *prim_dir, Paths out,
-                   int *out_id, float *out_t, int *live_count, hipStream_t st) {
-    k_expand<<<grid_for(total, 8192), kBlock, 0, st>>>(views, segs, n_segs, total, prim_list,
-                                                        prim_hit, prim_dir, out, out_id, out_t,
-                                                        live_count);
+void launch_expand(const ViewParams *views, const Segment *segs_all, const PassDesc *passes,
+                   const int *pass, int64_t max_total, const int *prim_list, const int2 *prim_hit,
+                   const float4 *prim_dir, Paths out, int *out_id, float *out_t, CountRef live,
+                   hipStream_t st) {
+    k_expand<<<grid_for(max_total, 8192), kBlock, 0, st>>>(views, segs_all, passes, pass,
+                                                            prim_list, prim_hit, prim_dir, out,
+                                                            out_id, out_t, live);
 }
 
-void launch_shade(const DevScene &sc, const ViewParams *views, int depth_index, const int *n_in,
-                  Paths in, const int *hit_id, const float *hit_t, Paths out, int *n_out,
+void launch_shade(const DevScene &sc, const ViewParams *views, int depth_index, CountRef n_in,
+                  Paths in, const int *hit_id, const float *hit_t, Paths out, CountRef n_out,
                   float *accum, int grid, hipStream_t st) {
     k_shade<<<grid, kBlock, 0, st>>>(sc, views, depth_index, n_in, in, hit_id, hit_t, out, n_out,
                                      accum);
 }
+
+__global__ void k_next_pass(int *pass) { *pass += 1; }
+
+void launch_next_pass(int *pass, hipStream_t st) { k_next_pass<<<1, 1, 0, st>>>(pass); }
 
 void launch_resolve(const ViewParams *views, int view_id, int width, int height, int ssx, int ssy,
                     float brightness, const float *accum, uchar4 *rgba, hipStream_t st) {

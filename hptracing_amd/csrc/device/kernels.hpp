@@ -82,6 +82,26 @@ struct WorkCounter {
     int32_t done;
 };
 
+// A live-path count in device memory.  Within a render the counts form a
+// table [pass][depth + 1]; the pass is read from device memory (`pass`), so
+// one captured launch sequence serves every pass.
+struct CountRef {
+    int *base;
+    const int *pass;   // nullptr: base[offset]
+    int stride;
+    int offset;
+    __device__ __forceinline__ int *ptr() const {
+        return base + (pass ? (*pass) * stride : 0) + offset;
+    }
+};
+
+// The segments and ray total of one pass: segs[seg_begin .. seg_begin+n_segs).
+struct PassDesc {
+    int32_t seg_begin;
+    int32_t n_segs;
+    int64_t total;
+};
+
 constexpr int kBlock = 256;
 
 }  // namespace hpt

@@ -9,19 +9,21 @@
 namespace hpt {
 
 void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const float4 *org4,
-                 const float4 *dir4, const int *n_dev, int64_t n_host, WorkCounter *wc,
+                 const float4 *dir4, CountRef n_dev, int64_t n_host, WorkCounter *wc,
                  int *out_id, float *out_t, int *counts, unsigned long long *totals, int grid,
                  hipStream_t st);
 void launch_camera(const ViewParams *views, int view_id, int64_t n, Paths out, hipStream_t st);
 void launch_primary(const ViewParams *views, int view_id, int64_t n, const int *hit_id,
                     const float *hit_t, float *accum, int2 *prim_hit, int *prim_list,
                     int *prim_count, hipStream_t st);
-void launch_expand(const ViewParams *views, const Segment *segs, int n_segs, int64_t total,
-                   const int *prim_list, const int2 *prim_hit, const float4 *prim_dir, Paths out,
-                   int *out_id, float *out_t, int *live_count, hipStream_t st);
-void launch_shade(const DevScene &sc, const ViewParams *views, int depth_index, const int *n_in,
-                  Paths in, const int *hit_id, const float *hit_t, Paths out, int *n_out,
+void launch_expand(const ViewParams *views, const Segment *segs_all, const PassDesc *passes,
+                   const int *pass, int64_t max_total, const int *prim_list, const int2 *prim_hit,
+                   const float4 *prim_dir, Paths out, int *out_id, float *out_t, CountRef live,
+                   hipStream_t st);
+void launch_shade(const DevScene &sc, const ViewParams *views, int depth_index, CountRef n_in,
+                  Paths in, const int *hit_id, const float *hit_t, Paths out, CountRef n_out,
                   float *accum, int grid, hipStream_t st);
+void launch_next_pass(int *pass, hipStream_t st);
 void launch_resolve(const ViewParams *views, int view_id, int width, int height, int ssx, int ssy,
                     float brightness, const float *accum, uchar4 *rgba, hipStream_t st);
 int occupancy_blocks_cast();

@@ -75,6 +75,23 @@ class TraceRunner:
         self.last_stats = stats
         return (rgba, accum) if return_accum else rgba
 
+    def render_batch(self, views, return_accum=False, use_graph=True):
+        """Render several views in shared launches (hpt_batch_render).
+        Returns a list of RGBA images (and accumulators)."""
+        n = len(views)
+        arr = (N.View * n)(*views)
+        imgs = [np.zeros((v.sample_y, v.sample_x, 4), np.uint8) for v in views]
+        accs = [np.zeros((v.sample_x * v.supersample_x * v.sample_y * v.supersample_y, 3),
+                         np.float32) for v in views] if return_accum else None
+        rgba_ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in imgs])
+        acc_ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in accs]) if return_accum else None
+        stats = N.RenderStats()
+        flags = 0 if use_graph else N.HPT_RENDER_NO_GRAPH
+        N.check_device(N.device().hpt_batch_render(self._h, arr, n, rgba_ptrs, acc_ptrs,
+                                                   C.byref(stats), flags))
+        self.last_stats = stats
+        return (imgs, accs) if return_accum else imgs
+
     def cast(self, org, dir, counted=False):
         """Cast rays given as (n,3) float32 arrays (numpy or torch, any device);
         returns (tri_id int32, t float32[, counts (n,4) int32]) as torch CUDA
@@ -102,6 +119,57 @@ class TraceRunner:
     def close(self):
         if self._h:
             N.device().hpt_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RenderServer:
+    """Continuous-batching server over a resident scene (hpt_server_*).
+
+    ``submit(view)`` queues a request and returns its id; ``wait(id)``
+    blocks until the image is in the returned buffer.  Requests submitted
+    while others render join them at the next pass."""
+
+    def __init__(self, runner: TraceRunner, device_budget_bytes: int = 4 << 30):
+        self.runner = runner
+        self._h = C.c_void_p()
+        opts = N.ServerOpts(device_budget_bytes)
+        N.check_device(N.device().hpt_server_create(runner.handle, C.byref(opts),
+                                                    C.byref(self._h)))
+        self._buffers = {}
+
+    def submit(self, v: N.View, return_accum=False) -> int:
+        img = np.zeros((v.sample_y, v.sample_x, 4), np.uint8)
+        acc = np.zeros((v.sample_x * v.supersample_x * v.sample_y * v.supersample_y, 3),
+                       np.float32) if return_accum else None
+        rid = C.c_int64()
+        N.check_device(N.device().hpt_server_submit(
+            self._h, C.byref(v), img.ctypes.data, acc.ctypes.data if acc is not None else None,
+            C.byref(rid)))
+        self._buffers[rid.value] = (img, acc)
+        return rid.value
+
+    def wait(self, rid: int, timeout_s: float = -1.0):
+        """Returns (rgba, accum or None, info dict)."""
+        info = N.RequestInfo()
+        N.check_device(N.device().hpt_server_wait(self._h, rid, timeout_s, C.byref(info)))
+        img, acc = self._buffers.pop(rid)
+        return img, acc, {"queue_ms": info.queue_ms, "latency_ms": info.latency_ms,
+                          "camera_rays": info.camera_rays}
+
+    def stats(self) -> dict:
+        st = N.ServerStats()
+        N.check_device(N.device().hpt_server_stats_get(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in N.ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            N.device().hpt_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -204,6 +204,9 @@ typedef struct {
 
 #define HPT_RENDER_TIME_KERNELS 1u
 #define HPT_RENDER_COUNT_WORK 2u
+/* Enqueue every pass directly instead of replaying one captured hipGraph
+ * of the pass (the graph is also skipped while kernels are timed). */
+#define HPT_RENDER_NO_GRAPH 4u
 
 /* Render one view.  `rgba` (host, W*H*4 bytes, may be NULL) receives the
  * tone-mapped image exactly as get_image writes it (kernel.cl:615-652;
@@ -215,6 +218,64 @@ int hpt_render(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
 /* hpt_render with HPT_RENDER_* flags. */
 int hpt_render_ex(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
                   float *accum, hpt_render_stats *stats, unsigned flags);
+
+/* Render several views of one scene together (no reference counterpart; the
+ * reference renders one view per TraceRunner::run call).  The camera rays
+ * of all views are cast in one launch, and every pass launches the
+ * expand / cast / shade sequence once over the live paths of every view
+ * whose spp exceeds the pass index, so views of uneven size and spp share
+ * the machine.  Each view's result equals its hpt_render result bit for bit
+ * (paths carry their own random streams).  rgba[k] / accum[k] may be NULL;
+ * `stats` sums over the batch. */
+int hpt_batch_render(hpt_scene *scene, const hpt_view *views, int n_views,
+                     uint8_t *const *rgba, float *const *accum,
+                     hpt_render_stats *stats, unsigned flags);
+
+/* ------------------------------------------------------------- serving */
+/* Continuous-batching render server over one resident scene (no reference
+ * counterpart; the reference's GUI re-renders one view at a time on a
+ * resident scene, hptracing.cpp:124-130).  A worker thread admits queued
+ * requests in arrival order whenever their camera rays fit in the device
+ * arena (sized from device_budget_bytes), and runs one pass at a time over
+ * every admitted request, so requests that arrive while others render join
+ * at the next pass.  A request's result equals its hpt_render result bit
+ * for bit. */
+typedef struct hpt_server hpt_server;
+
+typedef struct {
+    int64_t device_budget_bytes; /* arena for in-flight requests; <= 0: 4 GiB */
+} hpt_server_opts;
+
+typedef struct {
+    int64_t id;
+    int32_t status;     /* 0 queued, 1 rendering, 2 done, 3 failed */
+    double queue_ms;    /* submit -> admission */
+    double latency_ms;  /* submit -> result in the caller's buffers */
+    int64_t camera_rays;
+} hpt_request_info;
+
+typedef struct {
+    int64_t queued, active, completed;
+    int64_t macro_passes;  /* passes run over the admitted set */
+    int64_t rays_cast;     /* primaries + continuation rays */
+    double busy_seconds;   /* worker time spent rendering */
+    int64_t capacity_rays; /* camera rays the arena holds */
+} hpt_server_stats;
+
+int hpt_server_create(hpt_scene *scene, const hpt_server_opts *opts,
+                      hpt_server **out);
+/* Queue a request.  rgba (W*H*4 bytes) and accum (may be NULL) are written
+ * when it completes and must stay valid until hpt_server_wait returns for
+ * it.  HPT_ERR_BUDGET if the request can never fit the arena. */
+int hpt_server_submit(hpt_server *server, const hpt_view *view, uint8_t *rgba,
+                      float *accum, int64_t *request_id);
+/* Wait up to timeout_s seconds (< 0: forever).  HPT_ERR_TIMEOUT if still
+ * running; otherwise the request is retired and its id forgotten. */
+int hpt_server_wait(hpt_server *server, int64_t request_id, double timeout_s,
+                    hpt_request_info *info);
+int hpt_server_stats_get(hpt_server *server, hpt_server_stats *out);
+/* Finish every queued and running request, then stop the worker. */
+int hpt_server_destroy(hpt_server *server);
 
 #ifdef __cplusplus
 }
