@@ -125,6 +125,11 @@ HPT_RENDER_NO_GRAPH = 4
 HPT_CAST_COUNTS = 5
 
 
+class Tuning(C.Structure):
+    _fields_ = [("fast_division", C.c_int32), ("cast_blocks_per_cu", C.c_int32),
+                ("shade_blocks_per_cu", C.c_int32)]
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("device_budget_bytes", C.c_int64)]
 
@@ -188,6 +193,8 @@ def device() -> C.CDLL:
                                       C.c_uint]
         lib.hpt_batch_render.argtypes = [vp, C.POINTER(View), C.c_int, C.POINTER(vp),
                                          C.POINTER(vp), C.POINTER(RenderStats), C.c_uint]
+        lib.hpt_scene_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        lib.hpt_scene_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         lib.hpt_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
         lib.hpt_server_submit.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(C.c_int64)]
         lib.hpt_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(RequestInfo)]

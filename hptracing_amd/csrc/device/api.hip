@@ -178,10 +178,12 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.n_lights = (int32_t)d->n_lights;
         ds.textures = s->textures.as<uchar4>();
         ds.n_textures = (int32_t)d->n_textures;
+        ds.fast_div = 1;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
                   "hipDeviceGetAttribute");
+        s->n_cus = std::max(1, cus);
         s->cast_grid = std::max(1, cus * std::max(1, hpt::occupancy_blocks_cast()));
         s->shade_grid = std::max(1, cus * std::max(1, hpt::occupancy_blocks_shade()));
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -208,6 +210,28 @@ int hpt_scene_stats_get(const hpt_scene *s, hpt_scene_stats *out) {
         out->n_triangles = s->n_triangles;
         out->n_textures = s->n_textures;
         out->device_bytes = s->device_bytes;
+    });
+}
+
+int hpt_scene_get_tuning(const hpt_scene *s, hpt_tuning *out) {
+    return guarded([&] {
+        if (!s || !out) fail(HPT_ERR_INVALID, "null argument");
+        out->fast_division = s->dev.fast_div;
+        out->cast_blocks_per_cu = s->cast_grid / std::max(1, s->n_cus);
+        out->shade_blocks_per_cu = s->shade_grid / std::max(1, s->n_cus);
+    });
+}
+
+int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
+    return guarded([&] {
+        if (!s || !t) fail(HPT_ERR_INVALID, "null argument");
+        if (t->cast_blocks_per_cu < 1 || t->cast_blocks_per_cu > 64 ||
+            t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64)
+            fail(HPT_ERR_INVALID, "blocks per CU must be in [1, 64]");
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->dev.fast_div = t->fast_division ? 1 : 0;
+        s->cast_grid = t->cast_blocks_per_cu * s->n_cus;
+        s->shade_grid = t->shade_blocks_per_cu * s->n_cus;
     });
 }
 

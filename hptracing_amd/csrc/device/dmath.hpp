@@ -55,6 +55,47 @@ __device__ __forceinline__ bool box_hit(V3 bs, V3 be, V3 sp, V3 d) {
     return max_of_mins <= min_of_maxs;
 }
 
+// Quotient (s - p) / d through the ray's reciprocal y = 1/d (correctly
+// rounded), with Markstein's correction: q0 = a*y, r = fma(-q0, d, a) (exact),
+// q = fma(r, y, q0) equals the correctly rounded a/d whenever y is finite and
+// q0 is a normal number well inside the exponent range (binary32, round to
+// nearest).  For d = +-0 the product a*y = a*(+-inf) already is a/d.  Any
+// other case falls back to the division, so the result is always a/d.
+__device__ __forceinline__ float quot(float a, float d, float y) {
+    const float q0 = a * y;
+    if (!isfinite(y)) return q0;                   // d = +-0: a * +-inf == a / +-0
+    const float aq = fabsf(q0);
+    if (!(aq <= 0x1p100f && (aq >= 0x1p-100f || aq == 0.f))) return a / d;
+    const float r = fmaf(-q0, d, a);
+    return fmaf(r, y, q0);
+}
+
+// box_hit with the six quotients formed by `quot`; `inv` = (1/d.x, 1/d.y,
+// 1/d.z).  Same result as box_hit for every input (rays whose direction
+// has a subnormal component must use box_hit: their reciprocal overflows).
+__device__ __forceinline__ bool box_hit_rcp(V3 bs, V3 be, V3 sp, V3 d, V3 inv) {
+    if (sp.x >= bs.x && sp.x <= be.x && sp.y >= bs.y && sp.y <= be.y && sp.z >= bs.z &&
+        sp.z <= be.z)
+        return true;
+    const float tx0 = quot(bs.x - sp.x, d.x, inv.x), tx1 = quot(be.x - sp.x, d.x, inv.x);
+    const float ty0 = quot(bs.y - sp.y, d.y, inv.y), ty1 = quot(be.y - sp.y, d.y, inv.y);
+    const float tz0 = quot(bs.z - sp.z, d.z, inv.z), tz1 = quot(be.z - sp.z, d.z, inv.z);
+    const float max_of_mins = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    const float min_of_maxs = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    if (min_of_maxs <= 0) return false;
+    return max_of_mins <= min_of_maxs;
+}
+
+// Directions whose reciprocal is not a normal number (subnormal, huge or
+// non-finite components) take the plain division.
+__device__ __forceinline__ bool needs_division(V3 d) {
+    auto bad = [](float c) {
+        const float a = fabsf(c);
+        return !(a <= 0x1p100f) || (c != 0.f && a < 0x1p-100f);
+    };
+    return bad(d.x) || bad(d.y) || bad(d.z);
+}
+
 // Moller-Trumbore `_single_intersect` (kernel.cl:85-109) on a triangle
 // record that stores pa and the edges e1 = pb - pa, e2 = pc - pa (the
 // same float subtractions the kernel performs).  The degenerate guard of

@@ -171,7 +171,7 @@ struct hpt_scene {
         counters;
     int64_t device_bytes = 0;
     int64_t n_nodes = 0, n_entries = 0, n_triangles = 0, n_textures = 0;
-    int cast_grid = 0, shade_grid = 0;
+    int cast_grid = 0, shade_grid = 0, n_cus = 1;
     std::atomic<int> next_counter{0};
     std::mutex mu;
     hpt::Workspace ws;

@@ -26,20 +26,25 @@ namespace hpt {
 // which is the order the reference tests its collected list in; the walk
 // stops after the 512th non-empty leaf because the reference collects no
 // more (kernel.cl:146) and so can change nothing after it.
-template <bool COUNT>
+// FAST forms the slab quotients through the ray's reciprocal direction with
+// an exact correction (quot in dmath.hpp): same decisions, fewer divisions.
+template <bool COUNT, bool FAST>
 __device__ __forceinline__ int trace_ray(const DevScene &sc, V3 sp, V3 d, float &t_out,
                                          int *cnt) {
     int idx = 0, leaves = 0, best_id = -1;
     float best = -42.f;
     int visits = 0, tested = 0, dropped = 0, visits_cap = -1;
     const int nn = sc.n_nodes;
+    const bool rcp = FAST && !needs_division(d);
+    const V3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
     while (idx < nn) {
         const float4 lo = sc.nodes[2 * idx];
         const float4 hi = sc.nodes[2 * idx + 1];
         const int a = __float_as_int(lo.w);
         const int b = __float_as_int(hi.w);
         if (COUNT) ++visits;
-        const bool hit = box_hit(xyz(lo), xyz(hi), sp, d);
+        const bool hit = rcp ? box_hit_rcp(xyz(lo), xyz(hi), sp, d, inv)
+                             : box_hit(xyz(lo), xyz(hi), sp, d);
         if (b < 0) {
             idx = hit ? idx + 1 : a;
             continue;
@@ -94,7 +99,7 @@ __device__ __forceinline__ void wave_done(WorkCounter *wc) {
 // With COUNT, per-ray counters go to `counts` (HPT_CAST_COUNTS ints per ray)
 // when it is given, and their wave sums (nodes to the cap, leaves,
 // triangles) to the 64-bit `totals` when that is given.
-template <bool COUNT, bool PATHS>
+template <bool COUNT, bool PATHS, bool FAST>
 __global__ __launch_bounds__(kBlock) void k_cast(DevScene sc, const float *org3, const float *dir3,
                                                  const float4 *org4, const float4 *dir4,
                                                  CountRef n_dev, int64_t n_host,
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_cast(DevScene sc, const float *org3,
             }
             float t;
             int c[HPT_CAST_COUNTS];
-            const int id = trace_ray<COUNT>(sc, sp, d, t, c);
+            const int id = trace_ray<COUNT, FAST>(sc, sp, d, t, c);
             out_id[i] = id;
             out_t[i] = t;
             if (COUNT) {
@@ -531,17 +536,19 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
                  int *out_id, float *out_t, int *counts, unsigned long long *totals, int grid,
                  hipStream_t st) {
     const bool paths = org4 != nullptr;
-    if (counts || totals) {
-        if (paths)
-            k_cast<true, true><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, out_t, counts, totals);
-        else
-            k_cast<true, false><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, out_t, counts, totals);
+    const bool count = counts || totals;
+    const bool fast = sc.fast_div != 0;
+#define HPT_CAST(C, P, F)                                                                       \
+    k_cast<C, P, F><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, \
+                                             out_t, counts, totals)
+    if (count) {
+        if (paths) { if (fast) HPT_CAST(true, true, true); else HPT_CAST(true, true, false); }
+        else { if (fast) HPT_CAST(true, false, true); else HPT_CAST(true, false, false); }
     } else {
-        if (paths)
-            k_cast<false, true><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, out_t, counts, totals);
-        else
-            k_cast<false, false><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, out_t, counts, totals);
+        if (paths) { if (fast) HPT_CAST(false, true, true); else HPT_CAST(false, true, false); }
+        else { if (fast) HPT_CAST(false, false, true); else HPT_CAST(false, false, false); }
     }
+#undef HPT_CAST
 }
 
 void launch_camera(const ViewParams *views, int view_id, int64_t n, Paths out, hipStream_t st) {
@@ -583,7 +590,7 @@ void launch_resolve(const ViewParams *views, int view_id, int width, int height,
 
 int occupancy_blocks_cast() {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cast<false, true>, kBlock, 0) != hipSuccess) b = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cast<false, true, true>, kBlock, 0) != hipSuccess) b = 4;
     return b;
 }
 

@@ -41,6 +41,7 @@ struct DevScene {
     int32_t n_lights;
     const uchar4 *textures;
     int32_t n_textures;
+    int32_t fast_div;  // slab quotients via corrected reciprocals (same results)
 };
 
 struct Paths {

@@ -54,6 +54,17 @@ class TraceRunner:
     def handle(self):
         return self._h
 
+    def get_tuning(self) -> dict:
+        t = N.Tuning()
+        N.check_device(N.device().hpt_scene_get_tuning(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in N.Tuning._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = N.Tuning(**cur)
+        N.check_device(N.device().hpt_scene_set_tuning(self._h, C.byref(t)))
+
     def run(self, view_p, top_dir, right_dir, width, height, sample_x, sample_y,
             supersample_x=1, supersample_y=1, background_color=(0, 0, 0), sample=10,
             depth=6, disable_diffuse=0, brightness=1.0, seed=1, return_accum=False):

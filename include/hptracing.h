@@ -144,6 +144,17 @@ typedef struct {
 } hpt_scene_stats;
 int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 
+/* Launch and arithmetic choices of a scene's kernels.  None changes a
+ * result: every setting gives the same bits.  Defaults are set by
+ * hpt_scene_create (fast_division = 1, grids = occupancy x CUs). */
+typedef struct {
+    int32_t fast_division;       /* slab quotients via corrected reciprocals */
+    int32_t cast_blocks_per_cu;  /* persistent grid of the cast kernel      */
+    int32_t shade_blocks_per_cu; /* persistent grid of the shade kernel     */
+} hpt_tuning;
+int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
+int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);
+
 /* Ray cast, the parity entry point of `kdtree_intersect`
  * (kernel.cl:111-208): for each of n rays (origin, direction as xyz float
  * triples in DEVICE memory) write the index of the hit triangle (-1 on a

@@ -12,14 +12,21 @@ from hptracing_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _runner(kd):
+def _runner(kd, fast=1):
     from hptracing_amd.runner import TraceRunner
-    return TraceRunner(kd)
+    r = TraceRunner(kd)
+    r.set_tuning(fast_division=fast)
+    return r
 
 
-def _check(kd, org, d, oracle_mod, counted=False):
+@pytest.fixture(params=[1, 0], ids=["rcp-division", "ieee-division"])
+def fast(request):
+    return request.param
+
+
+def _check(kd, org, d, oracle_mod, counted=False, fast=1):
     import torch
-    r = _runner(kd)
+    r = _runner(kd, fast)
     s = oracle_mod.Scene(kd)
     if counted:
         ids, t, cnt = r.cast(org, d, counted=True)
@@ -37,9 +44,9 @@ def _check(kd, org, d, oracle_mod, counted=False):
     return ids, t
 
 
-def test_cast_room_random(room, oracle_mod):
+def test_cast_room_random(room, oracle_mod, fast):
     org, d = scenes.random_rays(50000, (0, 0, -2), (10, 10, 10), seed=11)
-    ids, _ = _check(room, org, d, oracle_mod)
+    ids, _ = _check(room, org, d, oracle_mod, fast=fast)
     assert (ids >= 0).mean() > 0.5
 
 
@@ -48,12 +55,12 @@ def test_cast_room_counted(room, oracle_mod):
     _check(room, org, d, oracle_mod, counted=True)
 
 
-def test_cast_soup(soup_small, oracle_mod):
+def test_cast_soup(soup_small, oracle_mod, fast):
     org, d = scenes.random_rays(50000, (0, 0, 0), (1, 1, 1), seed=13)
-    _check(soup_small, org, d, oracle_mod)
+    _check(soup_small, org, d, oracle_mod, fast=fast)
 
 
-def test_cast_axis_aligned_and_degenerate(room, oracle_mod):
+def test_cast_axis_aligned_and_degenerate(room, oracle_mod, fast):
     # Zero direction components (division by zero in the slab test), rays
     # starting inside boxes and on faces, and a zero direction.
     rng = np.random.default_rng(5)
@@ -65,7 +72,9 @@ def test_cast_axis_aligned_and_degenerate(room, oracle_mod):
     d[np.arange(4096), axis] = rng.choice([-1.0, 1.0], 4096)
     d[2048:] = rng.normal(size=(2048, 3)).astype(np.float32)
     d[3000:3100] = 0.0
-    _check(room, org, d, oracle_mod)
+    d[3100:3200, 0] = 1e-40          # subnormal component: plain division path
+    d[3200:3300] *= 1e30             # huge components
+    _check(room, org, d, oracle_mod, fast=fast)
 
 
 def test_cast_empty(room):
