@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=4,
                     help="passes of the kernel-timed and work-counted renders")
     ap.add_argument("--scene-dir", default=os.path.join(tempfile.gettempdir(), "hpt_scenes"))
+    ap.add_argument("--batch-views", type=int, default=64, help="0 skips the batching leg")
+    ap.add_argument("--serve-requests", type=int, default=40, help="0 skips the serving leg")
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[2.0, 8.0])
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -120,6 +123,15 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(kd, view, args.steps, args.cpu_seconds, R)
 
+    batching = None
+    if args.batch_views > 0:
+        batching = bench_batching(runner, args.batch_views, args.depth)
+
+    serving = None
+    if args.serve_requests > 0:
+        del runner
+        serving = bench_serving(args)
+
     out = {
         "metric": "Mrays/s (all bounces)",
         "value": round(value, 3),
@@ -158,8 +170,91 @@ def main():
             "algorithmic_bytes_per_launch": int(cast_bytes / max(1, tst.cast_launches)),
         },
         "cpu_baseline": cpu,
+        "batching": batching,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_batching(runner, n_views, depth):
+    """64 uneven views of the same scene in one batch (shared launches, one
+    captured pass graph) against the same views rendered one at a time."""
+    import math
+
+    import numpy as np
+
+    from hptracing_amd import scenes
+    from hptracing_amd.runner import make_view
+
+    rng = np.random.default_rng(7)
+    views = []
+    for i in range(n_views):
+        w = int(rng.integers(128, 1025))
+        h = int(rng.integers(128, 769))
+        spp = int(rng.integers(4, 65))
+        pos = rng.uniform([1.5, 1.5, 1.5], [scenes.ROOM - 1.5, scenes.ROOM - 2.0, scenes.ROOM - 1.5])
+        yaw = rng.uniform(0, 2 * math.pi)
+        pitch = rng.uniform(-0.3, 0.3)
+        up = (math.sin(pitch) * -math.sin(yaw), math.cos(pitch), math.sin(pitch) * math.cos(yaw))
+        right = (math.cos(yaw), 0.0, math.sin(yaw))
+        views.append(make_view(tuple(pos), up, right, w / h, 1.0, w, h, 1, 1, (0, 0, 0), spp,
+                               depth, 0, 1.0, seed=500 + i))
+    runner.render_batch(views[:2])  # warm the path
+    t0 = time.perf_counter()
+    runner.render_batch(views)
+    batch_s = time.perf_counter() - t0
+    rays = int(runner.last_stats.rays_cast)
+    t0 = time.perf_counter()
+    seq_rays = 0
+    for v in views:
+        runner.render(v)
+        seq_rays += int(runner.last_stats.rays_cast)
+    seq_s = time.perf_counter() - t0
+    log(f"[bench] batching {n_views} views: batch {batch_s:.2f}s ({rays / batch_s / 1e6:.1f} "
+        f"Mrays/s), one at a time {seq_s:.2f}s ({seq_rays / seq_s / 1e6:.1f} Mrays/s)")
+    return {"views": n_views, "batch_s": round(batch_s, 3), "sequential_s": round(seq_s, 3),
+            "batch_mrays_s": round(rays / batch_s / 1e6, 2),
+            "sequential_mrays_s": round(seq_rays / seq_s / 1e6, 2),
+            "speedup": round(seq_s / batch_s, 3)}
+
+
+def bench_serving(args):
+    """Serving: Poisson arrivals of the request mix on the 2M-triangle scene,
+    p50/p99 latency per arrival rate, and served throughput against the same
+    requests rendered one at a time (identical images, so equal error)."""
+    from hptracing_amd import scenes, serving
+    from hptracing_amd.runner import TraceRunner
+    from hptracing_amd.scene import KDTree
+
+    t0 = time.time()
+    path = scenes.cached(args.scene_dir, scenes.write_full, 0, water_side=1000)
+    kd = KDTree(path, os.path.dirname(path) + "/")
+    log(f"[bench] serving scene: {kd.n_triangles} tris, KD {kd.build_seconds:.1f}s, "
+        f"total {time.time() - t0:.1f}s")
+    runner = TraceRunner(kd)
+    views = serving.request_views(args.serve_requests, seed=3, depth=args.depth)
+    runner.render(views[0])  # warm
+    seq_s, seq_rays = serving.one_at_a_time(runner, views)
+    seq_mrays = seq_rays / seq_s / 1e6
+    log(f"[bench] one at a time: {len(views)} requests {seq_s:.2f}s {seq_mrays:.1f} Mrays/s")
+    rates = []
+    for rate in args.serve_rates:
+        r = serving.serve(runner, views, rate, seed=11)
+        log(f"[bench] served at {rate}/s: p50 {r.p50_ms:.1f} ms p99 {r.p99_ms:.1f} ms, "
+            f"{r.mrays_s:.1f} Mrays/s over {r.wall_s:.2f}s, {r.macro_passes} passes")
+        rates.append({"arrivals_per_s": rate, "p50_ms": round(r.p50_ms, 2),
+                      "p99_ms": round(r.p99_ms, 2), "mean_ms": round(r.mean_ms, 2),
+                      "wall_s": round(r.wall_s, 3), "mrays_s": round(r.mrays_s, 2)})
+    burst = serving.serve(runner, views, 0.0, seed=11)
+    log(f"[bench] served all at once: {burst.wall_s:.2f}s {burst.mrays_s:.1f} Mrays/s, "
+        f"p99 {burst.p99_ms:.1f} ms")
+    return {"scene_triangles": kd.n_triangles, "requests": len(views),
+            "one_at_a_time_s": round(seq_s, 3), "one_at_a_time_mrays_s": round(seq_mrays, 2),
+            "served_burst_s": round(burst.wall_s, 3),
+            "served_burst_mrays_s": round(burst.mrays_s, 2),
+            "served_vs_one_at_a_time": round(seq_s / burst.wall_s, 3),
+            "burst_p50_ms": round(burst.p50_ms, 2), "burst_p99_ms": round(burst.p99_ms, 2),
+            "poisson": rates}
 
 
 def cpu_baseline(kd, view, spp, target_s, R):

@@ -176,10 +176,24 @@ def host() -> C.CDLL:
     return _host
 
 
+def _init_torch_runtime_first() -> None:
+    """torch carries its own copy of the HIP runtime.  That copy reports no
+    GPU when another HIP runtime in the same process has opened the device
+    first, while the reverse order works; so when torch is importable its
+    runtime is initialised before libhptracing.so makes its first call."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
 def device() -> C.CDLL:
     """libhptracing.so with argument types declared."""
     global _device
     if _device is None:
+        _init_torch_runtime_first()
         lib = _load(DEVICE_LIB)
         vp, i64 = C.c_void_p, C.c_int64
         lib.hpt_last_error.restype = C.c_char_p
