@@ -127,7 +127,7 @@ HPT_CAST_COUNTS = 5
 
 class Tuning(C.Structure):
     _fields_ = [("fast_division", C.c_int32), ("cast_blocks_per_cu", C.c_int32),
-                ("shade_blocks_per_cu", C.c_int32)]
+                ("shade_blocks_per_cu", C.c_int32), ("traversal", C.c_int32)]
 
 
 class ServerOpts(C.Structure):

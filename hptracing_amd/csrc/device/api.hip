@@ -179,6 +179,7 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.textures = s->textures.as<uchar4>();
         ds.n_textures = (int32_t)d->n_textures;
         ds.fast_div = 1;
+        ds.traversal = 1;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
@@ -219,6 +220,7 @@ int hpt_scene_get_tuning(const hpt_scene *s, hpt_tuning *out) {
         out->fast_division = s->dev.fast_div;
         out->cast_blocks_per_cu = s->cast_grid / std::max(1, s->n_cus);
         out->shade_blocks_per_cu = s->shade_grid / std::max(1, s->n_cus);
+        out->traversal = s->dev.traversal;
     });
 }
 
@@ -230,6 +232,7 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
             fail(HPT_ERR_INVALID, "blocks per CU must be in [1, 64]");
         std::lock_guard<std::mutex> lock(s->mu);
         s->dev.fast_div = t->fast_division ? 1 : 0;
+        s->dev.traversal = t->traversal ? 1 : 0;
         s->cast_grid = t->cast_blocks_per_cu * s->n_cus;
         s->shade_grid = t->shade_blocks_per_cu * s->n_cus;
     });

@@ -83,6 +83,57 @@ __device__ __forceinline__ int trace_ray(const DevScene &sc, V3 sp, V3 d, float 
     return best_id;
 }
 
+// The same search organised as alternating phases for the whole wave: every
+// lane walks nodes until it reaches a non-empty leaf (or the end of the
+// tree), then all lanes test their leaf's triangles together.  A lane tests
+// its leaves in the order it reaches them, as above, so the result is the
+// same; the triangle loop runs once per leaf round instead of on almost
+// every node step of some lane.
+template <bool FAST>
+__device__ __forceinline__ int trace_ray_ww(const DevScene &sc, V3 sp, V3 d, float &t_out) {
+    int idx = 0, leaves = 0, best_id = -1;
+    float best = -42.f;
+    const int nn = sc.n_nodes;
+    const bool rcp = FAST && !needs_division(d);
+    const V3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    bool done = nn <= 0;
+    int pcnt = 0, poff = 0;
+    for (;;) {
+        while (!done && pcnt == 0) {
+            const float4 lo = sc.nodes[2 * idx];
+            const float4 hi = sc.nodes[2 * idx + 1];
+            const int a = __float_as_int(lo.w);
+            const int b = __float_as_int(hi.w);
+            const bool hit = rcp ? box_hit_rcp(xyz(lo), xyz(hi), sp, d, inv)
+                                 : box_hit(xyz(lo), xyz(hi), sp, d);
+            if (b < 0) {
+                idx = hit ? idx + 1 : a;
+            } else {
+                if (hit && a > 0) {
+                    pcnt = a;
+                    poff = b;
+                }
+                ++idx;
+            }
+            if (idx >= nn) done = true;
+        }
+        if (pcnt == 0) break;
+        const float4 *rec = sc.tris + 3 * (int64_t)poff;
+        for (int k = 0; k < pcnt; ++k) {
+            const float4 r0 = rec[3 * k], r1 = rec[3 * k + 1], r2 = rec[3 * k + 2];
+            const float r = tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
+            if (r > 0 && (best < 0 || r < best)) {
+                best = r;
+                best_id = __float_as_int(r0.w);
+            }
+        }
+        pcnt = 0;
+        if (++leaves == kMaxMatchData) done = true;
+    }
+    t_out = best;
+    return best_id;
+}
+
 __device__ __forceinline__ void wave_done(WorkCounter *wc) {
     if ((threadIdx.x & 63) == 0) {
         const int total_waves = (int)(gridDim.x * (blockDim.x / 64));
@@ -125,7 +176,8 @@ __global__ __launch_bounds__(kBlock) void k_cast(DevScene sc, const float *org3,
             }
             float t;
             int c[HPT_CAST_COUNTS];
-            const int id = trace_ray<COUNT, FAST>(sc, sp, d, t, c);
+            const int id = (!COUNT && sc.traversal == 1) ? trace_ray_ww<FAST>(sc, sp, d, t)
+                                                         : trace_ray<COUNT, FAST>(sc, sp, d, t, c);
             out_id[i] = id;
             out_t[i] = t;
             if (COUNT) {

@@ -42,6 +42,7 @@ struct DevScene {
     const uchar4 *textures;
     int32_t n_textures;
     int32_t fast_div;  // slab quotients via corrected reciprocals (same results)
+    int32_t traversal; // 0: node and leaf work interleaved per lane; 1: wave phases
 };
 
 struct Paths {

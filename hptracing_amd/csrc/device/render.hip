@@ -287,4 +287,78 @@ int hpt_batch_render(hpt_scene *s, const hpt_view *views, int n_views, uint8_t *
     return guarded([&] { render_batch(s, views, n_views, rgba, accum, stats, flags); });
 }
 
+int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *chosen) {
+    return guarded([&] {
+        if (!s || !view) fail(HPT_ERR_INVALID, "null argument");
+        validate_view(view);
+        // A reduced copy of the view: same camera and depth, one pass, the
+        // image shrunk until it has at most max_rays camera rays.
+        hpt_view v = *view;
+        v.sample = 1;
+        if (max_rays <= 0) max_rays = 1 << 20;
+        while ((int64_t)v.sample_x * v.supersample_x * v.sample_y * v.supersample_y > max_rays &&
+               v.sample_x > 8 && v.sample_y > 8) {
+            v.sample_x /= 2;
+            v.sample_y /= 2;
+        }
+        hpt_tuning base;
+        {
+            std::lock_guard<std::mutex> lock(s->mu);
+            base.fast_division = s->dev.fast_div;
+            base.traversal = s->dev.traversal;
+            base.cast_blocks_per_cu = s->cast_grid / s->n_cus;
+            base.shade_blocks_per_cu = s->shade_grid / s->n_cus;
+        }
+        auto apply = [&](const hpt_tuning &t) {
+            std::lock_guard<std::mutex> lock(s->mu);
+            s->dev.fast_div = t.fast_division;
+            s->dev.traversal = t.traversal;
+            s->cast_grid = std::max(1, t.cast_blocks_per_cu) * s->n_cus;
+            s->shade_grid = std::max(1, t.shade_blocks_per_cu) * s->n_cus;
+        };
+        auto measure = [&](const hpt_tuning &t, bool cast) {
+            apply(t);
+            hpt_render_stats st;
+            double best = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                render_batch(s, &v, 1, nullptr, nullptr, &st, HPT_RENDER_TIME_KERNELS);
+                best = std::min(best, cast ? st.cast_ms : st.shade_ms);
+            }
+            return best;
+        };
+        measure(base, true);  // warm caches and code
+        hpt_tuning best = base;
+        double best_ms = 1e30;
+        const int occ = std::max(1, base.cast_blocks_per_cu);
+        for (int trav = 0; trav <= 1; ++trav)
+            for (int fast = 0; fast <= 1; ++fast)
+                for (int mult : {1, 2}) {
+                    hpt_tuning t = base;
+                    t.traversal = trav;
+                    t.fast_division = fast;
+                    t.cast_blocks_per_cu = std::min(64, occ * mult);
+                    const double ms = measure(t, true);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = t;
+                    }
+                }
+        const int socc = std::max(1, base.shade_blocks_per_cu);
+        double best_shade = 1e30;
+        int best_sbpc = socc;
+        for (int sb : {std::max(1, socc / 2), socc, std::min(64, socc * 2)}) {
+            hpt_tuning t = best;
+            t.shade_blocks_per_cu = sb;
+            const double ms = measure(t, false);
+            if (ms < best_shade) {
+                best_shade = ms;
+                best_sbpc = sb;
+            }
+        }
+        best.shade_blocks_per_cu = best_sbpc;
+        apply(best);
+        if (chosen) *chosen = best;
+    });
+}
+
 }  // extern "C"

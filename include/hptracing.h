@@ -146,11 +146,14 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
- * hpt_scene_create (fast_division = 1, grids = occupancy x CUs). */
+ * hpt_scene_create (fast_division = 1, traversal = 1, grids = occupancy x
+ * CUs). */
 typedef struct {
     int32_t fast_division;       /* slab quotients via corrected reciprocals */
     int32_t cast_blocks_per_cu;  /* persistent grid of the cast kernel      */
     int32_t shade_blocks_per_cu; /* persistent grid of the shade kernel     */
+    int32_t traversal;           /* 0: node and leaf work interleaved per lane,
+                                    1: whole-wave node and leaf phases      */
 } hpt_tuning;
 int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);
@@ -229,6 +232,13 @@ int hpt_render(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
 /* hpt_render with HPT_RENDER_* flags. */
 int hpt_render_ex(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
                   float *accum, hpt_render_stats *stats, unsigned flags);
+
+/* Time the candidate tunings (hpt_tuning) on a one-pass render of `view`
+ * shrunk to at most max_rays camera rays (<= 0: 2^20), keep the fastest
+ * cast and shade settings on the scene and return them in `chosen` (may be
+ * NULL).  No reference counterpart. */
+int hpt_autotune(hpt_scene *scene, const hpt_view *view, int max_rays,
+                 hpt_tuning *chosen);
 
 /* Render several views of one scene together (no reference counterpart; the
  * reference renders one view per TraceRunner::run call).  The camera rays

@@ -12,19 +12,21 @@ from hptracing_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _runner(kd, fast=1):
+def _runner(kd, fast=(1, 1)):
     from hptracing_amd.runner import TraceRunner
     r = TraceRunner(kd)
-    r.set_tuning(fast_division=fast)
+    r.set_tuning(fast_division=fast[0], traversal=fast[1])
     return r
 
 
-@pytest.fixture(params=[1, 0], ids=["rcp-division", "ieee-division"])
+@pytest.fixture(params=[(1, 1), (0, 1), (1, 0), (0, 0)],
+                ids=["rcp-phases", "ieee-phases", "rcp-interleaved", "ieee-interleaved"])
 def fast(request):
+    """(fast_division, traversal) tunings: every one must give the same bits."""
     return request.param
 
 
-def _check(kd, org, d, oracle_mod, counted=False, fast=1):
+def _check(kd, org, d, oracle_mod, counted=False, fast=(1, 1)):
     import torch
     r = _runner(kd, fast)
     s = oracle_mod.Scene(kd)
