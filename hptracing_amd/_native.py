@@ -127,7 +127,8 @@ HPT_CAST_COUNTS = 5
 
 class Tuning(C.Structure):
     _fields_ = [("fast_division", C.c_int32), ("cast_blocks_per_cu", C.c_int32),
-                ("shade_blocks_per_cu", C.c_int32), ("traversal", C.c_int32)]
+                ("shade_blocks_per_cu", C.c_int32), ("traversal", C.c_int32),
+                ("refill_threshold", C.c_int32), ("node_steps", C.c_int32)]
 
 
 class ServerOpts(C.Structure):
@@ -207,6 +208,8 @@ def device() -> C.CDLL:
                                       C.c_uint]
         lib.hpt_batch_render.argtypes = [vp, C.POINTER(View), C.c_int, C.POINTER(vp),
                                          C.POINTER(vp), C.POINTER(RenderStats), C.c_uint]
+        lib.hpt_scene_set_arithmetic.argtypes = [vp, C.c_int]
+        lib.hpt_autotune.argtypes = [vp, C.POINTER(View), C.c_int, C.POINTER(Tuning)]
         lib.hpt_scene_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         lib.hpt_scene_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         lib.hpt_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]

@@ -178,8 +178,11 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.n_lights = (int32_t)d->n_lights;
         ds.textures = s->textures.as<uchar4>();
         ds.n_textures = (int32_t)d->n_textures;
-        ds.fast_div = 1;
-        ds.traversal = 1;
+        ds.fast_div = 0;  // measured slower than the hardware division sequence
+        ds.traversal = 2;
+        ds.arith_ocl = 0;
+        ds.refill = 16;
+        ds.node_steps = 16;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
@@ -221,6 +224,8 @@ int hpt_scene_get_tuning(const hpt_scene *s, hpt_tuning *out) {
         out->cast_blocks_per_cu = s->cast_grid / std::max(1, s->n_cus);
         out->shade_blocks_per_cu = s->shade_grid / std::max(1, s->n_cus);
         out->traversal = s->dev.traversal;
+        out->refill_threshold = s->dev.refill;
+        out->node_steps = s->dev.node_steps;
     });
 }
 
@@ -228,13 +233,27 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
     return guarded([&] {
         if (!s || !t) fail(HPT_ERR_INVALID, "null argument");
         if (t->cast_blocks_per_cu < 1 || t->cast_blocks_per_cu > 64 ||
-            t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64)
-            fail(HPT_ERR_INVALID, "blocks per CU must be in [1, 64]");
+            t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64 ||
+            t->traversal < 0 || t->traversal > 2 || t->refill_threshold < 1 ||
+            t->refill_threshold > 64 || t->node_steps < 1 || t->node_steps > 4096)
+            fail(HPT_ERR_INVALID, "tuning value out of range");
         std::lock_guard<std::mutex> lock(s->mu);
         s->dev.fast_div = t->fast_division ? 1 : 0;
-        s->dev.traversal = t->traversal ? 1 : 0;
+        s->dev.traversal = t->traversal;
+        s->dev.refill = t->refill_threshold;
+        s->dev.node_steps = t->node_steps;
         s->cast_grid = t->cast_blocks_per_cu * s->n_cus;
         s->shade_grid = t->shade_blocks_per_cu * s->n_cus;
+    });
+}
+
+int hpt_scene_set_arithmetic(hpt_scene *s, int mode) {
+    return guarded([&] {
+        if (!s) fail(HPT_ERR_INVALID, "null scene");
+        if (mode != HPT_ARITH_LITERAL && mode != HPT_ARITH_ROCM_OPENCL)
+            fail(HPT_ERR_INVALID, "unknown arithmetic mode");
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->dev.arith_ocl = mode == HPT_ARITH_ROCM_OPENCL ? 1 : 0;
     });
 }
 

@@ -54,7 +54,8 @@ __device__ __forceinline__ int trace_ray(const DevScene &sc, V3 sp, V3 d, float 
                 const float4 *rec = sc.tris + 3 * (int64_t)b;
                 for (int k = 0; k < a; ++k) {
                     const float4 r0 = rec[3 * k], r1 = rec[3 * k + 1], r2 = rec[3 * k + 2];
-                    const float r = tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
+                    const float r = sc.arith_ocl ? tri_hit_ocl(sp, d, xyz(r0), xyz(r1), xyz(r2))
+                                              : tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
                     if (r > 0 && (best < 0 || r < best)) {
                         best = r;
                         best_id = __float_as_int(r0.w);
@@ -121,7 +122,8 @@ __device__ __forceinline__ int trace_ray_ww(const DevScene &sc, V3 sp, V3 d, flo
         const float4 *rec = sc.tris + 3 * (int64_t)poff;
         for (int k = 0; k < pcnt; ++k) {
             const float4 r0 = rec[3 * k], r1 = rec[3 * k + 1], r2 = rec[3 * k + 2];
-            const float r = tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
+            const float r = sc.arith_ocl ? tri_hit_ocl(sp, d, xyz(r0), xyz(r1), xyz(r2))
+                                              : tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
             if (r > 0 && (best < 0 || r < best)) {
                 best = r;
                 best_id = __float_as_int(r0.w);
@@ -146,6 +148,109 @@ __device__ __forceinline__ void wave_done(WorkCounter *wc) {
     }
 }
 
+// The search with per-lane refill.  Per-ray costs differ by orders of
+// magnitude (a ray skimming the water grid crosses thousands of nodes), so a
+// wave that waits for its slowest lane idles most of its lanes.  Here every
+// lane carries its own ray through a sequence of rounds -- at most
+// `node_steps` node visits until it reaches a non-empty leaf, then that
+// leaf's triangles -- and a lane whose ray is finished takes the next ray
+// from the shared counter as soon as `refill` lanes of the wave are idle.
+// Each ray's leaves are still tested in the order its walk reaches them, so
+// results are those of trace_ray.
+template <bool PATHS, bool FAST>
+__device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
+                                            const float *dir3, const float4 *org4,
+                                            const float4 *dir4, int64_t n, WorkCounter *wc,
+                                            int *out_id, float *out_t) {
+    const int lane = threadIdx.x & 63;
+    const int nn = sc.n_nodes;
+    const int threshold = sc.refill < 1 ? 1 : (sc.refill > 64 ? 64 : sc.refill);
+    const int node_steps = sc.node_steps < 1 ? 1 : sc.node_steps;
+    int64_t ray = -1;
+    V3 sp = mk(0.f, 0.f, 0.f), d = sp, inv = sp;
+    bool rcp = false, done = true;
+    int idx = 0, leaves = 0, best_id = -1, pcnt = 0, poff = 0;
+    float best = -42.f;
+    bool exhausted = false;
+    for (;;) {
+        const bool idle = ray < 0;
+        const uint64_t m = __ballot(idle);
+        const int n_idle = __popcll(m);
+        if (!exhausted && n_idle > 0 && (n_idle >= threshold || n_idle == 64)) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&wc->next, n_idle);
+            base = __shfl(base, 0, 64);
+            if ((int64_t)base + n_idle >= n) exhausted = true;
+            if (idle) {
+                const int64_t my = (int64_t)base + __popcll(m & ((1ULL << lane) - 1ULL));
+                if (my < n) {
+                    ray = my;
+                    if (PATHS) {
+                        sp = xyz(org4[my]);
+                        d = xyz(dir4[my]);
+                    } else {
+                        sp = mk(org3[3 * my], org3[3 * my + 1], org3[3 * my + 2]);
+                        d = mk(dir3[3 * my], dir3[3 * my + 1], dir3[3 * my + 2]);
+                    }
+                    rcp = FAST && !needs_division(d);
+                    inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+                    idx = 0;
+                    leaves = 0;
+                    best = -42.f;
+                    best_id = -1;
+                    pcnt = 0;
+                    done = nn <= 0;
+                }
+            }
+        }
+        if (__ballot(ray >= 0) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        for (int k = 0; k < node_steps; ++k) {
+            const bool walk = ray >= 0 && !done && pcnt == 0;
+            if (__ballot(walk) == 0) break;
+            if (walk) {
+                const float4 lo = sc.nodes[2 * idx];
+                const float4 hi = sc.nodes[2 * idx + 1];
+                const int a = __float_as_int(lo.w);
+                const int b = __float_as_int(hi.w);
+                const bool hit = rcp ? box_hit_rcp(xyz(lo), xyz(hi), sp, d, inv)
+                                     : box_hit(xyz(lo), xyz(hi), sp, d);
+                if (b < 0) {
+                    idx = hit ? idx + 1 : a;
+                } else {
+                    if (hit && a > 0) {
+                        pcnt = a;
+                        poff = b;
+                    }
+                    ++idx;
+                }
+                if (idx >= nn) done = true;
+            }
+        }
+        if (ray >= 0 && pcnt > 0) {
+            const float4 *rec = sc.tris + 3 * (int64_t)poff;
+            for (int k = 0; k < pcnt; ++k) {
+                const float4 r0 = rec[3 * k], r1 = rec[3 * k + 1], r2 = rec[3 * k + 2];
+                const float r = sc.arith_ocl ? tri_hit_ocl(sp, d, xyz(r0), xyz(r1), xyz(r2))
+                                             : tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
+                if (r > 0 && (best < 0 || r < best)) {
+                    best = r;
+                    best_id = __float_as_int(r0.w);
+                }
+            }
+            pcnt = 0;
+            if (++leaves == kMaxMatchData) done = true;
+        }
+        if (ray >= 0 && done && pcnt == 0) {
+            out_id[ray] = best_id;
+            out_t[ray] = best;
+            ray = -1;
+        }
+    }
+}
+
 // Rays given as xyz triples (hpt_cast) or as path state (render).
 // With COUNT, per-ray counters go to `counts` (HPT_CAST_COUNTS ints per ray)
 // when it is given, and their wave sums (nodes to the cap, leaves,
@@ -157,6 +262,11 @@ __global__ __launch_bounds__(kBlock) void k_cast(DevScene sc, const float *org3,
                                                  WorkCounter *wc, int *out_id, float *out_t,
                                                  int *counts, unsigned long long *totals) {
     const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
+    if (!COUNT && sc.traversal == 2) {
+        cast_refill<PATHS, FAST>(sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
+        wave_done(wc);
+        return;
+    }
     const int lane = threadIdx.x & 63;
     unsigned long long tot[3] = {0, 0, 0};
     for (;;) {

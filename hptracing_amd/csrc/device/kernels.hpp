@@ -42,7 +42,10 @@ struct DevScene {
     const uchar4 *textures;
     int32_t n_textures;
     int32_t fast_div;  // slab quotients via corrected reciprocals (same results)
-    int32_t traversal; // 0: node and leaf work interleaved per lane; 1: wave phases
+    int32_t traversal; // 0: interleaved per lane; 1: wave phases; 2: phases + lane refill
+    int32_t arith_ocl; // dot/cross as ROCm's OpenCL builtins (HPT_ARITH_ROCM_OPENCL)
+    int32_t refill;     // traversal 2: idle lanes that trigger a refill
+    int32_t node_steps; // traversal 2: node visits per round
 };
 
 struct Paths {

@@ -306,6 +306,8 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             std::lock_guard<std::mutex> lock(s->mu);
             base.fast_division = s->dev.fast_div;
             base.traversal = s->dev.traversal;
+            base.refill_threshold = s->dev.refill;
+            base.node_steps = s->dev.node_steps;
             base.cast_blocks_per_cu = s->cast_grid / s->n_cus;
             base.shade_blocks_per_cu = s->shade_grid / s->n_cus;
         }
@@ -313,6 +315,8 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             std::lock_guard<std::mutex> lock(s->mu);
             s->dev.fast_div = t.fast_division;
             s->dev.traversal = t.traversal;
+            s->dev.refill = t.refill_threshold;
+            s->dev.node_steps = t.node_steps;
             s->cast_grid = std::max(1, t.cast_blocks_per_cu) * s->n_cus;
             s->shade_grid = std::max(1, t.shade_blocks_per_cu) * s->n_cus;
         };
@@ -329,20 +333,36 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
         measure(base, true);  // warm caches and code
         hpt_tuning best = base;
         double best_ms = 1e30;
-        const int occ = std::max(1, base.cast_blocks_per_cu);
-        for (int trav = 0; trav <= 1; ++trav)
-            for (int fast = 0; fast <= 1; ++fast)
-                for (int mult : {1, 2}) {
-                    hpt_tuning t = base;
-                    t.traversal = trav;
-                    t.fast_division = fast;
-                    t.cast_blocks_per_cu = std::min(64, occ * mult);
-                    const double ms = measure(t, true);
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best = t;
-                    }
-                }
+        auto consider = [&](const hpt_tuning &t) {
+            const double ms = measure(t, true);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        };
+        // Traversal organisation and, for lane refill, its two thresholds.
+        for (int trav = 0; trav <= 1; ++trav) {
+            hpt_tuning t = base;
+            t.traversal = trav;
+            consider(t);
+        }
+        for (int refill : {8, 16, 32, 64})
+            for (int steps : {4, 16, 64}) {
+                hpt_tuning t = base;
+                t.traversal = 2;
+                t.refill_threshold = refill;
+                t.node_steps = steps;
+                consider(t);
+            }
+        // Division form and grid size around the best organisation.
+        {
+            hpt_tuning t = best;
+            t.fast_division = 1 - t.fast_division;
+            consider(t);
+            t = best;
+            t.cast_blocks_per_cu = std::min(64, 2 * std::max(1, best.cast_blocks_per_cu));
+            consider(t);
+        }
         const int socc = std::max(1, base.shade_blocks_per_cu);
         double best_shade = 1e30;
         int best_sbpc = socc;

@@ -65,6 +65,17 @@ class TraceRunner:
         t = N.Tuning(**cur)
         N.check_device(N.device().hpt_scene_set_tuning(self._h, C.byref(t)))
 
+    def autotune(self, v: N.View, max_rays: int = 1 << 20) -> dict:
+        """Pick the fastest tuning on a reduced one-pass render of `v`."""
+        t = N.Tuning()
+        N.check_device(N.device().hpt_autotune(self._h, C.byref(v), max_rays, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in N.Tuning._fields_}
+
+    def set_arithmetic(self, rocm_opencl: bool) -> None:
+        """dot/cross of the intersection test: literal (default) or the fused
+        forms of ROCm's OpenCL builtins."""
+        N.check_device(N.device().hpt_scene_set_arithmetic(self._h, 1 if rocm_opencl else 0))
+
     def run(self, view_p, top_dir, right_dir, width, height, sample_x, sample_y,
             supersample_x=1, supersample_y=1, background_color=(0, 0, 0), sample=10,
             depth=6, disable_diffuse=0, brightness=1.0, seed=1, return_accum=False):

@@ -146,17 +146,32 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
- * hpt_scene_create (fast_division = 1, traversal = 1, grids = occupancy x
- * CUs). */
+ * hpt_scene_create (fast_division = 0, traversal = 2, refill_threshold = 16,
+ * node_steps = 16, grids = occupancy x CUs). */
 typedef struct {
     int32_t fast_division;       /* slab quotients via corrected reciprocals */
     int32_t cast_blocks_per_cu;  /* persistent grid of the cast kernel      */
     int32_t shade_blocks_per_cu; /* persistent grid of the shade kernel     */
     int32_t traversal;           /* 0: node and leaf work interleaved per lane,
-                                    1: whole-wave node and leaf phases      */
+                                    1: whole-wave node and leaf phases,
+                                    2: phases with per-lane ray refill      */
+    int32_t refill_threshold;    /* traversal 2: idle lanes (1..64) that make
+                                    a wave fetch new rays                   */
+    int32_t node_steps;          /* traversal 2: node visits per round      */
 } hpt_tuning;
 int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);
+
+/* Evaluation of OpenCL's dot() and cross() in the intersection test.
+ * LITERAL (default): the textbook formulas with every product and sum
+ * rounded, (a.x*b.x + a.y*b.y) + a.z*b.z and a.y*b.z - a.z*b.y ..., which
+ * is what kernel.cl writes.  ROCM_OPENCL: the fused forms ROCm's OpenCL
+ * device library uses for those builtins, so that hit distances equal the
+ * reference kernel's bit for bit when it is compiled by that library.  Hit
+ * triangles agree in both modes except on exact ties. */
+#define HPT_ARITH_LITERAL 0
+#define HPT_ARITH_ROCM_OPENCL 1
+int hpt_scene_set_arithmetic(hpt_scene *scene, int mode);
 
 /* Ray cast, the parity entry point of `kdtree_intersect`
  * (kernel.cl:111-208): for each of n rays (origin, direction as xyz float

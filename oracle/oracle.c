@@ -64,21 +64,47 @@ int orc_box_intersect(const float bs[3], const float be[3], const float sp[3], c
 }
 
 /* ------------------------------------------------------------ kernel.cl:85-109 */
+/* dot/cross of the intersection test: ORC_ARITH_LITERAL rounds every product
+ * and sum; ORC_ARITH_ROCM_OPENCL fuses them as ROCm's OpenCL device library
+ * does (tests/golden/rocm_ocl_builtins/samples.npz holds the library's
+ * outputs). */
+static int g_arith = ORC_ARITH_LITERAL;
+void orc_set_arithmetic(int mode) { g_arith = mode; }
+
+static inline float dot_ocl(v3 a, v3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+static inline v3 cross_ocl(v3 a, v3 b) {
+    return mk(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)),
+              fmaf(a.x, b.y, -(a.y * b.x)));
+}
+
 static inline float single_intersect(v3 sp, v3 d, v3 pa, v3 pb, v3 pc) {
+    const int ocl = g_arith == ORC_ARITH_ROCM_OPENCL;
     v3 e1 = sub(pb, pa), e2 = sub(pc, pa);
-    v3 P = cross(d, e2);
-    float det = dot(e1, P);
+    v3 P = ocl ? cross_ocl(d, e2) : cross(d, e2);
+    float det = ocl ? dot_ocl(e1, P) : dot(e1, P);
     /* `det > EPSILON && det < EPSILON` (kernel.cl:93) can never hold. */
     float inv_det = 1.f / det;
     v3 T = sub(sp, pa);
-    float u = dot(T, P) * inv_det;
+    float u = (ocl ? dot_ocl(T, P) : dot(T, P)) * inv_det;
     if (u < 0.f || u > 1.f) return -1;
-    v3 Q = cross(T, e1);
-    float v = dot(d, Q) * inv_det;
+    v3 Q = ocl ? cross_ocl(T, e1) : cross(T, e1);
+    float v = (ocl ? dot_ocl(d, Q) : dot(d, Q)) * inv_det;
     if (v < 0.f || u + v > 1.f) return -1;
-    float t = dot(e2, Q) * inv_det;
+    float t = (ocl ? dot_ocl(e2, Q) : dot(e2, Q)) * inv_det;
     if (t > 0) return t;
     return -1;
+}
+
+void orc_dot_cross(int mode, const float *a, const float *b, int64_t n, float *out) {
+    for (int64_t i = 0; i < n; ++i) {
+        v3 x = ld3(a + 3 * i), y = ld3(b + 3 * i);
+        float d = mode == ORC_ARITH_ROCM_OPENCL ? dot_ocl(x, y) : dot(x, y);
+        v3 c = mode == ORC_ARITH_ROCM_OPENCL ? cross_ocl(x, y) : cross(x, y);
+        out[4 * i] = d;
+        out[4 * i + 1] = c.x;
+        out[4 * i + 2] = c.y;
+        out[4 * i + 3] = c.z;
+    }
 }
 
 float orc_single_intersect(const float sp[3], const float d[3], const float pa[3],

@@ -68,6 +68,16 @@ typedef struct {
 
 #define ORC_MAX_MATCH_DATA 512
 
+/* Evaluation of dot/cross in the intersection test (process-wide setting;
+ * set it before casting).  LITERAL: every product and sum rounded, as
+ * kernel.cl writes them.  ROCM_OPENCL: the fused forms of ROCm's OpenCL
+ * builtins.  orc_dot_cross evaluates both for a pin against the library's
+ * recorded outputs (4 floats per pair: dot, cross.xyz). */
+#define ORC_ARITH_LITERAL 0
+#define ORC_ARITH_ROCM_OPENCL 1
+void orc_set_arithmetic(int mode);
+void orc_dot_cross(int mode, const float *a, const float *b, int64_t n, float *out);
+
 /* ---- kernel.cl device functions ---- */
 int orc_box_intersect(const float box_start[3], const float box_end[3],
                       const float start[3], const float dir[3]);

@@ -78,6 +78,10 @@ def lib() -> C.CDLL:
         L.orc_render_partial.restype = i64
         L.orc_path_seed.argtypes = [C.c_uint64, i64, C.c_int32]
         L.orc_path_seed.restype = C.c_uint64
+        L.orc_set_arithmetic.argtypes = [C.c_int]
+        L.orc_set_arithmetic.restype = None
+        L.orc_dot_cross.argtypes = [C.c_int, vp, vp, i64, vp]
+        L.orc_dot_cross.restype = None
         _lib = L
     return _lib
 
@@ -216,3 +220,20 @@ def get_image(accum, width, height, ssx, ssy, spp, brightness):
 
 def path_seed(seed: int, ray: int, pass_: int) -> int:
     return int(lib().orc_path_seed(seed, ray, pass_))
+
+
+ARITH_LITERAL = 0
+ARITH_ROCM_OPENCL = 1
+
+
+def set_arithmetic(mode: int) -> None:
+    """dot/cross evaluation of the intersection test (process-wide)."""
+    lib().orc_set_arithmetic(mode)
+
+
+def dot_cross(mode: int, a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.empty((len(a), 4), np.float32)
+    lib().orc_dot_cross(mode, _p(a), _p(b), len(a), _p(out))
+    return out[:, 0], out[:, 1:]

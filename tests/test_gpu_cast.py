@@ -12,21 +12,25 @@ from hptracing_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _runner(kd, fast=(1, 1)):
+def _runner(kd, fast=(0, 2, 16, 16)):
     from hptracing_amd.runner import TraceRunner
     r = TraceRunner(kd)
-    r.set_tuning(fast_division=fast[0], traversal=fast[1])
+    r.set_tuning(fast_division=fast[0], traversal=fast[1], refill_threshold=fast[2],
+                 node_steps=fast[3])
     return r
 
 
-@pytest.fixture(params=[(1, 1), (0, 1), (1, 0), (0, 0)],
-                ids=["rcp-phases", "ieee-phases", "rcp-interleaved", "ieee-interleaved"])
+@pytest.fixture(params=[(0, 2, 16, 16), (1, 2, 1, 1), (0, 2, 64, 64), (1, 1, 16, 16),
+                        (0, 1, 16, 16), (1, 0, 16, 16), (0, 0, 16, 16)],
+                ids=["ieee-refill", "rcp-refill-every-step", "ieee-refill-64", "rcp-phases",
+                     "ieee-phases", "rcp-interleaved", "ieee-interleaved"])
 def fast(request):
-    """(fast_division, traversal) tunings: every one must give the same bits."""
+    """(fast_division, traversal, refill_threshold, node_steps) tunings:
+    every one must give the same bits."""
     return request.param
 
 
-def _check(kd, org, d, oracle_mod, counted=False, fast=(1, 1)):
+def _check(kd, org, d, oracle_mod, counted=False, fast=(0, 2, 16, 16)):
     import torch
     r = _runner(kd, fast)
     s = oracle_mod.Scene(kd)

@@ -118,6 +118,24 @@ def test_kd_cast_soup_matches_naive(soup_small):
     assert np.array_equal(t[ok].view(np.int32), nt[ok].view(np.int32))
 
 
+def test_rocm_opencl_builtin_forms():
+    """The oracle's ORC_ARITH_ROCM_OPENCL dot/cross reproduce the outputs
+    recorded from ROCm's OpenCL device library on gfx950 bit for bit; the
+    literal forms do not, so the fixture tells the two apart."""
+    import os
+    from conftest import GOLDEN
+    z = np.load(os.path.join(GOLDEN, "rocm_ocl_builtins", "samples.npz"))
+    d1, c1 = O.dot_cross(O.ARITH_ROCM_OPENCL, z["a"], z["b"])
+    assert np.array_equal(d1.view(np.int32), z["dot"].view(np.int32))
+    assert np.array_equal(c1.view(np.int32), z["cross"].view(np.int32))
+    d0, c0 = O.dot_cross(O.ARITH_LITERAL, z["a"], z["b"])
+    assert (d0.view(np.int32) != z["dot"].view(np.int32)).mean() > 0.1
+    assert (c0.view(np.int32) != z["cross"].view(np.int32)).mean() > 0.1
+    # the library's division is correctly rounded
+    q = (z["a"][:, 0] / z["b"][:, 0]).astype(np.float32)
+    assert np.array_equal(q.view(np.int32), z["div"].view(np.int32))
+
+
 # -------------------------------------------------------------------- RNG
 def test_lcg_matches_java_random():
     """The 48-bit LCG is java.util.Random's (same multiplier, increment and
