@@ -146,6 +146,11 @@ int64_t orc_render_partial(const orc_scene *s, const orc_view *v, int64_t ray_be
 
 uint64_t orc_path_seed(uint64_t seed, int64_t ray, int32_t pass);
 
+/* Float64 restatement (oracle64.c, liboracle64.so): pinhole renders with
+ * double sums and, optionally, sums of squared per-pass values. */
+int orc64_render(const orc_scene *s, const orc_view *v, int n_threads, double *accum,
+                 double *accum_sq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -231,6 +231,29 @@ def set_arithmetic(mode: int) -> None:
     lib().orc_set_arithmetic(mode)
 
 
+_lib64 = None
+
+
+def render64(scene: "Scene", view: OrcView, n_threads: int = 0):
+    """Float64 restatement (liboracle64.so): returns (sum, sum of squares)
+    of the per-pass radiance per camera ray, each (R, 3) float64."""
+    global _lib64
+    if _lib64 is None:
+        path = os.path.join(HERE, "liboracle64.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc64_render.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcView), C.c_int,
+                                   C.c_void_p, C.c_void_p]
+        _lib64 = L
+    R = view.sample_x * view.supersample_x * view.sample_y * view.supersample_y
+    s = np.zeros(R * 3, np.float64)
+    s2 = np.zeros(R * 3, np.float64)
+    if _lib64.orc64_render(C.byref(scene.c), C.byref(view), n_threads, _p(s), _p(s2)) != 0:
+        raise RuntimeError("orc64_render failed")
+    return s.reshape(-1, 3), s2.reshape(-1, 3)
+
+
 def dot_cross(mode: int, a, b):
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)

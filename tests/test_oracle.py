@@ -249,6 +249,24 @@ def test_wavefront_equals_depth_first(cornell):
     assert live[0] == 24 * 24
 
 
+def test_float_oracle_matches_float64_reference(room):
+    """The float32 oracle against the independent float64 restatement's
+    16384-spp render: >= 99% of pixel means within 4 standard errors."""
+    import os
+    from conftest import GOLDEN
+    ref = np.load(os.path.join(GOLDEN, "room_reference", "reference.npz"))
+    spp = 64
+    v = O.make_view(view=scenes.ROOM_VIEW["view_p"], up=scenes.ROOM_VIEW["top_dir"],
+                    right=scenes.ROOM_VIEW["right_dir"], width=1.0, height=1.0,
+                    sample_x=int(ref["width"]), sample_y=int(ref["height"]), sample=spp,
+                    depth=int(ref["depth"]), seed=4242)
+    acc, _, _ = O.Scene(room).render(v)
+    mean = acc.astype(np.float64) / spp
+    se = np.sqrt(ref["se"] ** 2 + (ref["std"] / np.sqrt(spp)) ** 2)
+    ok = (np.abs(mean - ref["mean"]) <= 4 * se) | (np.abs(mean - ref["mean"]) < 1e-9)
+    assert ok.mean() >= 0.99
+
+
 def test_seed_schemes_statistically_equivalent(cornell):
     """Per-path LCG streams (the GPU scheme) against the reference's slot
     seeds (v_seed[slot] + slot): per-pixel means agree within 4 standard
