@@ -14,6 +14,19 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
+def within_4se(mean, spp, ref):
+    """Image statistics against the float64 reference render
+    (golden/room_reference/reference.npz): the fraction of pixel channels
+    whose spp-sample mean lies within 4 combined standard errors of the
+    reference mean, and the RMSE.  Pixels of constant radiance (a direct view
+    of an emitter: reference std 0) differ only by float32 rounding, which the
+    1e-4 relative floor admits."""
+    se = np.sqrt(ref["se"] ** 2 + (ref["std"] / np.sqrt(spp)) ** 2)
+    diff = np.abs(mean - ref["mean"])
+    ok = diff <= 4 * se + 1e-4 * np.abs(ref["mean"]) + 1e-9
+    return float(ok.mean()), float(np.sqrt(np.mean((mean - ref["mean"]) ** 2)))
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 

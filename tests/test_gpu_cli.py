@@ -21,7 +21,7 @@ CLI = os.path.join(LIB_DIR, "hptracing")
 def read_png(path):
     data = open(path, "rb").read()
     assert data[:8] == b"\x89PNG\r\n\x1a\n"
-    pos, idat, w = 8, b"", h = None
+    pos, idat = 8, b""
     w = h = 0
     while pos < len(data):
         n, typ = struct.unpack(">I4s", data[pos:pos + 8])

@@ -17,11 +17,7 @@ REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "room_r
                    "reference.npz")
 
 
-def within_4se(mean, spp, ref):
-    se = np.sqrt(ref["se"] ** 2 + (ref["std"] / np.sqrt(spp)) ** 2)
-    z = np.abs(mean - ref["mean"]) / np.maximum(se, 1e-12)
-    ok = (z <= 4) | (np.abs(mean - ref["mean"]) < 1e-9)
-    return ok.mean(), float(np.sqrt(np.mean((mean - ref["mean"]) ** 2)))
+from conftest import within_4se
 
 
 @pytest.mark.parametrize("spp", [256, 16384])

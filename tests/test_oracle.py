@@ -251,20 +251,19 @@ def test_wavefront_equals_depth_first(cornell):
 
 def test_float_oracle_matches_float64_reference(room):
     """The float32 oracle against the independent float64 restatement's
-    16384-spp render: >= 99% of pixel means within 4 standard errors."""
+    16384-spp render at 256 spp: >= 99% of pixel means within 4 standard
+    errors (plus float32 rounding on constant-radiance pixels)."""
     import os
-    from conftest import GOLDEN
+    from conftest import GOLDEN, within_4se
     ref = np.load(os.path.join(GOLDEN, "room_reference", "reference.npz"))
-    spp = 64
+    spp = 256
     v = O.make_view(view=scenes.ROOM_VIEW["view_p"], up=scenes.ROOM_VIEW["top_dir"],
                     right=scenes.ROOM_VIEW["right_dir"], width=1.0, height=1.0,
                     sample_x=int(ref["width"]), sample_y=int(ref["height"]), sample=spp,
                     depth=int(ref["depth"]), seed=4242)
     acc, _, _ = O.Scene(room).render(v)
-    mean = acc.astype(np.float64) / spp
-    se = np.sqrt(ref["se"] ** 2 + (ref["std"] / np.sqrt(spp)) ** 2)
-    ok = (np.abs(mean - ref["mean"]) <= 4 * se) | (np.abs(mean - ref["mean"]) < 1e-9)
-    assert ok.mean() >= 0.99
+    frac, rmse = within_4se(acc.astype(np.float64) / spp, spp, ref)
+    assert frac >= 0.99, (frac, rmse)
 
 
 def test_seed_schemes_statistically_equivalent(cornell):
