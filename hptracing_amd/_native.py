@@ -128,7 +128,8 @@ HPT_CAST_COUNTS = 5
 class Tuning(C.Structure):
     _fields_ = [("fast_division", C.c_int32), ("cast_blocks_per_cu", C.c_int32),
                 ("shade_blocks_per_cu", C.c_int32), ("traversal", C.c_int32),
-                ("refill_threshold", C.c_int32), ("node_steps", C.c_int32)]
+                ("refill_threshold", C.c_int32), ("node_steps", C.c_int32),
+                ("fetch", C.c_int32)]
 
 
 class ServerOpts(C.Structure):

@@ -2,6 +2,8 @@
 // entry points (hpt_cast*), declared in include/hptracing.h.  Replaces the
 // constructor of the OpenCL host layer hp::TraceRunner
 // (src/hp/trace_runner.cpp:41-108) and exposes kdtree_intersect alone.
+#include <cmath>
+
 #include "internal.hpp"
 
 using namespace hpt;
@@ -90,6 +92,13 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         std::vector<float> pnodes(8 * nn);
         std::vector<float> recs;
         int64_t entries = 0;
+        // Box coordinates must be 0 or of magnitude in [2^-60, 2^80] for the
+        // reciprocal slab test to be certifiable (box_hit_rcp).
+        bool boxes_fast = true;
+        auto coord_ok = [](float c) {
+            const float a = std::fabs(c);
+            return a == 0.f || (a >= 0x1p-60f && a <= 0x1p80f);
+        };
         for (int64_t i = 0; i < nn; ++i) {
             const hpt_kd_node &nd = nodes[i];
             if (nd.child >= 0 && nd.child != i + 1)
@@ -136,6 +145,8 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
                 a = 0;  // empty leaf
                 b = 0;
             }
+            for (int k = 0; k < 3; ++k)
+                boxes_fast = boxes_fast && coord_ok(nd.box_start[k]) && coord_ok(nd.box_end[k]);
             float *p = &pnodes[8 * i];
             p[0] = nd.box_start[0]; p[1] = nd.box_start[1]; p[2] = nd.box_start[2];
             std::memcpy(&p[3], &a, 4);
@@ -178,11 +189,13 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.n_lights = (int32_t)d->n_lights;
         ds.textures = s->textures.as<uchar4>();
         ds.n_textures = (int32_t)d->n_textures;
-        ds.fast_div = 0;  // measured slower than the hardware division sequence
+        ds.fast_div = 1;
+        ds.boxes_fast = boxes_fast ? 1 : 0;
         ds.traversal = 2;
         ds.arith_ocl = 0;
         ds.refill = 16;
         ds.node_steps = 16;
+        ds.fetch = 4;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
@@ -226,6 +239,7 @@ int hpt_scene_get_tuning(const hpt_scene *s, hpt_tuning *out) {
         out->traversal = s->dev.traversal;
         out->refill_threshold = s->dev.refill;
         out->node_steps = s->dev.node_steps;
+        out->fetch = s->dev.fetch;
     });
 }
 
@@ -235,13 +249,15 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
         if (t->cast_blocks_per_cu < 1 || t->cast_blocks_per_cu > 64 ||
             t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64 ||
             t->traversal < 0 || t->traversal > 2 || t->refill_threshold < 1 ||
-            t->refill_threshold > 64 || t->node_steps < 1 || t->node_steps > 4096)
+            t->refill_threshold > 64 || t->node_steps < 1 || t->node_steps > 4096 ||
+            t->fetch < 0 || t->fetch > 7)
             fail(HPT_ERR_INVALID, "tuning value out of range");
         std::lock_guard<std::mutex> lock(s->mu);
         s->dev.fast_div = t->fast_division ? 1 : 0;
         s->dev.traversal = t->traversal;
         s->dev.refill = t->refill_threshold;
         s->dev.node_steps = t->node_steps;
+        s->dev.fetch = t->fetch;
         s->cast_grid = t->cast_blocks_per_cu * s->n_cus;
         s->shade_grid = t->shade_blocks_per_cu * s->n_cus;
     });

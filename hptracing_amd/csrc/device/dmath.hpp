@@ -55,45 +55,60 @@ __device__ __forceinline__ bool box_hit(V3 bs, V3 be, V3 sp, V3 d) {
     return max_of_mins <= min_of_maxs;
 }
 
-// Quotient (s - p) / d through the ray's reciprocal y = 1/d (correctly
-// rounded), with Markstein's correction: q0 = a*y, r = fma(-q0, d, a) (exact),
-// q = fma(r, y, q0) equals the correctly rounded a/d whenever y is finite and
-// q0 is a normal number well inside the exponent range (binary32, round to
-// nearest).  For d = +-0 the product a*y = a*(+-inf) already is a/d.  Any
-// other case falls back to the division, so the result is always a/d.
-__device__ __forceinline__ float quot(float a, float d, float y) {
-    const float q0 = a * y;
-    if (!isfinite(y)) return q0;                   // d = +-0: a * +-inf == a / +-0
-    const float aq = fabsf(q0);
-    if (!(aq <= 0x1p100f && (aq >= 0x1p-100f || aq == 0.f))) return a / d;
-    const float r = fmaf(-q0, d, a);
-    return fmaf(r, y, q0);
+// Slab test with the six quotients formed as (s - p) * y, y = v_rcp_f32(d)
+// (within 1 ulp of 1/d), certified against the rounding error so that the
+// answer is box_hit's for every input it is used on.
+//
+// Preconditions (checked by fast_box_ray per ray; the scene's box
+// coordinates are checked on upload): every direction component is 0 or
+// has magnitude in [2^-40, 2^40]; every origin and box coordinate is 0 or
+// has magnitude in [2^-60, 2^80].  Then every nonzero difference a = p - s
+// has |a| >= 2^-84 (both operands are multiples of 2^-84), so each quotient
+// is 0 or a normal number no larger than 2^121, and the approximate quotient
+// q' = fl(a * y) stays within relative 2^-21.9 of the correctly rounded
+// q = fl(a / d).  Zero, infinite and NaN quotients are identical in both
+// forms (a = 0, d = +-0: a * +-inf == a / +-0 with the same sign rules).
+//
+//  * "nearest exit <= 0" depends only on the sign class (negative, zero,
+//    positive, NaN) of each quotient, which q' shares with q: exact.
+//  * fminf/fmaxf return one of their inputs, and x -> x -+ e|x| is monotone
+//    for e < 1, so the entry (max of mins) and exit (min of maxs) formed
+//    from q' lie within relative e of those formed from q.  When they differ
+//    by more than 2^-16 of their magnitudes (far above 2 * 2^-21.9 plus the
+//    rounding of the check itself), or when either is infinite or NaN (then
+//    exact), their order is the exact order.  Otherwise the division form
+//    decides.
+__device__ __forceinline__ bool fast_box_ray(V3 sp, V3 d) {
+    auto dir_ok = [](float c) {
+        const float a = fabsf(c);
+        return a == 0.f || (a >= 0x1p-40f && a <= 0x1p40f);
+    };
+    auto org_ok = [](float c) {
+        const float a = fabsf(c);
+        return a == 0.f || (a >= 0x1p-60f && a <= 0x1p80f);
+    };
+    return dir_ok(d.x) && dir_ok(d.y) && dir_ok(d.z) && org_ok(sp.x) && org_ok(sp.y) &&
+           org_ok(sp.z);
 }
 
-// box_hit with the six quotients formed by `quot`; `inv` = (1/d.x, 1/d.y,
-// 1/d.z).  Same result as box_hit for every input (rays whose direction
-// has a subnormal component must use box_hit: their reciprocal overflows).
-__device__ __forceinline__ bool box_hit_rcp(V3 bs, V3 be, V3 sp, V3 d, V3 inv) {
+__device__ __forceinline__ V3 approx_rcp(V3 d) {
+    return mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
+}
+
+__device__ __forceinline__ bool box_hit_rcp(V3 bs, V3 be, V3 sp, V3 d, V3 y) {
     if (sp.x >= bs.x && sp.x <= be.x && sp.y >= bs.y && sp.y <= be.y && sp.z >= bs.z &&
         sp.z <= be.z)
         return true;
-    const float tx0 = quot(bs.x - sp.x, d.x, inv.x), tx1 = quot(be.x - sp.x, d.x, inv.x);
-    const float ty0 = quot(bs.y - sp.y, d.y, inv.y), ty1 = quot(be.y - sp.y, d.y, inv.y);
-    const float tz0 = quot(bs.z - sp.z, d.z, inv.z), tz1 = quot(be.z - sp.z, d.z, inv.z);
-    const float max_of_mins = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
-    const float min_of_maxs = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-    if (min_of_maxs <= 0) return false;
-    return max_of_mins <= min_of_maxs;
-}
-
-// Directions whose reciprocal is not a normal number (subnormal, huge or
-// non-finite components) take the plain division.
-__device__ __forceinline__ bool needs_division(V3 d) {
-    auto bad = [](float c) {
-        const float a = fabsf(c);
-        return !(a <= 0x1p100f) || (c != 0.f && a < 0x1p-100f);
-    };
-    return bad(d.x) || bad(d.y) || bad(d.z);
+    const float tx0 = (bs.x - sp.x) * y.x, tx1 = (be.x - sp.x) * y.x;
+    const float ty0 = (bs.y - sp.y) * y.y, ty1 = (be.y - sp.y) * y.y;
+    const float tz0 = (bs.z - sp.z) * y.z, tz1 = (be.z - sp.z) * y.z;
+    const float entry = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    const float exit = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    const float bound = (fabsf(entry) + fabsf(exit)) * 0x1p-16f;
+    const bool sure = fabsf(entry - exit) >= bound || !(bound < INFINITY);
+    const bool ahead = exit > 0.f;
+    if (!ahead || sure) return ahead && entry <= exit;
+    return box_hit(bs, be, sp, d);
 }
 
 // Moller-Trumbore `_single_intersect` (kernel.cl:85-109) on a triangle

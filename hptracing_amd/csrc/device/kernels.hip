@@ -26,8 +26,10 @@ namespace hpt {
 // which is the order the reference tests its collected list in; the walk
 // stops after the 512th non-empty leaf because the reference collects no
 // more (kernel.cl:146) and so can change nothing after it.
-// FAST forms the slab quotients through the ray's reciprocal direction with
-// an exact correction (quot in dmath.hpp): same decisions, fewer divisions.
+// FAST forms the slab quotients through the ray's approximate reciprocal
+// direction and certifies each decision against the rounding error, falling
+// back to the division when it cannot (box_hit_rcp): same decisions, almost
+// no divisions.
 template <bool COUNT, bool FAST>
 __device__ __forceinline__ int trace_ray(const DevScene &sc, V3 sp, V3 d, float &t_out,
                                          int *cnt) {
@@ -35,8 +37,8 @@ __device__ __forceinline__ int trace_ray(const DevScene &sc, V3 sp, V3 d, float 
     float best = -42.f;
     int visits = 0, tested = 0, dropped = 0, visits_cap = -1;
     const int nn = sc.n_nodes;
-    const bool rcp = FAST && !needs_division(d);
-    const V3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    const bool rcp = FAST && sc.boxes_fast && fast_box_ray(sp, d);
+    const V3 inv = approx_rcp(d);
     while (idx < nn) {
         const float4 lo = sc.nodes[2 * idx];
         const float4 hi = sc.nodes[2 * idx + 1];
@@ -95,8 +97,8 @@ __device__ __forceinline__ int trace_ray_ww(const DevScene &sc, V3 sp, V3 d, flo
     int idx = 0, leaves = 0, best_id = -1;
     float best = -42.f;
     const int nn = sc.n_nodes;
-    const bool rcp = FAST && !needs_division(d);
-    const V3 inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    const bool rcp = FAST && sc.boxes_fast && fast_box_ray(sp, d);
+    const V3 inv = approx_rcp(d);
     bool done = nn <= 0;
     int pcnt = 0, poff = 0;
     for (;;) {
@@ -148,6 +150,79 @@ __device__ __forceinline__ void wave_done(WorkCounter *wc) {
     }
 }
 
+// Wave-wide inclusive prefix sum and running maximum over lanes 0..63.
+__device__ __forceinline__ int wave_scan_add(int v, int lane) {
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_scan_max(int v, int lane) {
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(v, off, 64);
+        if (lane >= off) v = y > v ? y : v;
+    }
+    return v;
+}
+
+// One leaf round of the FLAT form (see cast_refill).  `cnt` is this lane's
+// pending triangle count (0 if none), `off` the first record of its leaf;
+// best / best_id are folded in place.  All 64 lanes must call it.
+__device__ __forceinline__ void flat_leaf_round(const DevScene &sc, int lane, int cnt, int off,
+                                                V3 sp, V3 d, float &best, int &best_id) {
+    __shared__ float s_r[kBlock];
+    __shared__ int s_id[kBlock];
+    __shared__ int s_mark[kBlock];
+    const int w = threadIdx.x & ~63;
+    const int incl = wave_scan_add(cnt, lane);
+    const int total = __shfl(incl, 63, 64);
+    const int excl = incl - cnt;
+    int carry = 0;  // owner of the position just before the current chunk
+    for (int base = 0; base < total; base += 64) {
+        // Owner of each position of the chunk: segment starts are marked
+        // with the owning lane and carried forward by a running maximum
+        // (owners are in lane order, their segments in position order).
+        s_mark[w + lane] = -1;
+        __builtin_amdgcn_wave_barrier();
+        if (cnt > 0 && excl >= base && excl < base + 64) s_mark[w + excl - base] = lane;
+        __builtin_amdgcn_wave_barrier();
+        int own = wave_scan_max(s_mark[w + lane], lane);
+        if (own < 0) own = carry;
+        carry = __shfl(own, 63, 64);
+        const int o_excl = __shfl(excl, own, 64);
+        const int o_off = __shfl(off, own, 64);
+        const V3 o_sp = mk(__shfl(sp.x, own, 64), __shfl(sp.y, own, 64), __shfl(sp.z, own, 64));
+        const V3 o_d = mk(__shfl(d.x, own, 64), __shfl(d.y, own, 64), __shfl(d.z, own, 64));
+        const int j = base + lane;
+        float r = -1.f;
+        int id = -1;
+        if (j < total) {
+            const float4 *rec = sc.tris + 3 * ((int64_t)o_off + (j - o_excl));
+            const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+            r = sc.arith_ocl ? tri_hit_ocl(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2))
+                             : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
+            id = __float_as_int(r0.w);
+        }
+        s_r[w + lane] = r;
+        s_id[w + lane] = id;
+        __builtin_amdgcn_wave_barrier();
+        // Each owner folds its part of the chunk in leaf order.
+        if (cnt > 0) {
+            const int lo = excl > base ? excl : base;
+            const int hi = excl + cnt < base + 64 ? excl + cnt : base + 64;
+            for (int p = lo; p < hi; ++p) {
+                const float rr = s_r[w + p - base];
+                if (rr > 0 && (best < 0 || rr < best)) {
+                    best = rr;
+                    best_id = s_id[w + p - base];
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // The search with per-lane refill.  Per-ray costs differ by orders of
 // magnitude (a ray skimming the water grid crosses thousands of nodes), so a
 // wave that waits for its slowest lane idles most of its lanes.  Here every
@@ -157,7 +232,24 @@ __device__ __forceinline__ void wave_done(WorkCounter *wc) {
 // from the shared counter as soon as `refill` lanes of the wave are idle.
 // Each ray's leaves are still tested in the order its walk reaches them, so
 // results are those of trace_ray.
-template <bool PATHS, bool FAST>
+//
+// Two fetch refinements, both leaving every result unchanged:
+//  PAIRS  nodes are fetched two at a time (idx and idx + 1, 64 contiguous
+//         bytes).  After a leaf, and after an interior node whose box is
+//         hit, the walk moves to idx + 1, so the second node is usually the
+//         next one visited and its box test needs no further memory trip;
+//         only a missed interior node (a jump to its escape) waits.
+//  PREF   a leaf's triangle records are fetched one triangle ahead, so the
+//         loads of the next record are in flight during the current test.
+//  FLAT   the leaf phase is spread over the whole wave: the (ray, triangle)
+//         pairs of every lane's pending leaf are numbered by a wave prefix
+//         sum and tested 64 at a time, one pair per lane, each tester
+//         reading its ray from the owning lane; the owner then folds its
+//         leaf's distances from LDS in the leaf's order, which is the
+//         reference's order, so best and ties come out the same.  A leaf
+//         round costs ceil(sum of counts / 64) tests instead of the
+//         largest count.
+template <bool PATHS, bool FAST, bool PAIRS, bool PREF, bool FLAT>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
                                             const float *dir3, const float4 *org4,
                                             const float4 *dir4, int64_t n, WorkCounter *wc,
@@ -172,6 +264,33 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
     int idx = 0, leaves = 0, best_id = -1, pcnt = 0, poff = 0;
     float best = -42.f;
     bool exhausted = false;
+    // One node visit: box test, then the next index (idx + 1 after a hit
+    // interior node or any leaf, the escape after a missed interior node),
+    // and a pending leaf when a non-empty leaf's box is hit.
+    auto visit = [&](float4 lo, float4 hi) {
+        const int a = __float_as_int(lo.w);
+        const int b = __float_as_int(hi.w);
+        const bool hit = rcp ? box_hit_rcp(xyz(lo), xyz(hi), sp, d, inv)
+                             : box_hit(xyz(lo), xyz(hi), sp, d);
+        if (b < 0) {
+            idx = hit ? idx + 1 : a;
+        } else {
+            if (hit && a > 0) {
+                pcnt = a;
+                poff = b;
+            }
+            ++idx;
+        }
+        if (idx >= nn) done = true;
+    };
+    auto test = [&](float4 r0, float4 r1, float4 r2) {
+        const float r = sc.arith_ocl ? tri_hit_ocl(sp, d, xyz(r0), xyz(r1), xyz(r2))
+                                     : tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
+        if (r > 0 && (best < 0 || r < best)) {
+            best = r;
+            best_id = __float_as_int(r0.w);
+        }
+    };
     for (;;) {
         const bool idle = ray < 0;
         const uint64_t m = __ballot(idle);
@@ -192,8 +311,8 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                         sp = mk(org3[3 * my], org3[3 * my + 1], org3[3 * my + 2]);
                         d = mk(dir3[3 * my], dir3[3 * my + 1], dir3[3 * my + 2]);
                     }
-                    rcp = FAST && !needs_division(d);
-                    inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+                    rcp = FAST && sc.boxes_fast && fast_box_ray(sp, d);
+                    inv = approx_rcp(d);
                     idx = 0;
                     leaves = 0;
                     best = -42.f;
@@ -213,32 +332,37 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             if (walk) {
                 const float4 lo = sc.nodes[2 * idx];
                 const float4 hi = sc.nodes[2 * idx + 1];
-                const int a = __float_as_int(lo.w);
-                const int b = __float_as_int(hi.w);
-                const bool hit = rcp ? box_hit_rcp(xyz(lo), xyz(hi), sp, d, inv)
-                                     : box_hit(xyz(lo), xyz(hi), sp, d);
-                if (b < 0) {
-                    idx = hit ? idx + 1 : a;
+                if (PAIRS) {
+                    const int nxt = idx + 1 < nn ? idx + 1 : idx;
+                    const float4 lo1 = sc.nodes[2 * nxt];
+                    const float4 hi1 = sc.nodes[2 * nxt + 1];
+                    visit(lo, hi);
+                    if (!done && pcnt == 0 && idx == nxt) visit(lo1, hi1);
                 } else {
-                    if (hit && a > 0) {
-                        pcnt = a;
-                        poff = b;
-                    }
-                    ++idx;
+                    visit(lo, hi);
                 }
-                if (idx >= nn) done = true;
             }
         }
-        if (ray >= 0 && pcnt > 0) {
+        if (FLAT) {
+            flat_leaf_round(sc, lane, ray >= 0 && pcnt > 0 ? pcnt : 0, poff, sp, d, best, best_id);
+            if (ray >= 0 && pcnt > 0) {
+                pcnt = 0;
+                if (++leaves == kMaxMatchData) done = true;
+            }
+        } else if (ray >= 0 && pcnt > 0) {
             const float4 *rec = sc.tris + 3 * (int64_t)poff;
-            for (int k = 0; k < pcnt; ++k) {
-                const float4 r0 = rec[3 * k], r1 = rec[3 * k + 1], r2 = rec[3 * k + 2];
-                const float r = sc.arith_ocl ? tri_hit_ocl(sp, d, xyz(r0), xyz(r1), xyz(r2))
-                                             : tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
-                if (r > 0 && (best < 0 || r < best)) {
-                    best = r;
-                    best_id = __float_as_int(r0.w);
+            if (PREF) {
+                float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+                for (int k = 0; k < pcnt; ++k) {
+                    const int kn = k + 1 < pcnt ? k + 1 : k;
+                    const float4 n0 = rec[3 * kn], n1 = rec[3 * kn + 1], n2 = rec[3 * kn + 2];
+                    test(r0, r1, r2);
+                    r0 = n0;
+                    r1 = n1;
+                    r2 = n2;
                 }
+            } else {
+                for (int k = 0; k < pcnt; ++k) test(rec[3 * k], rec[3 * k + 1], rec[3 * k + 2]);
             }
             pcnt = 0;
             if (++leaves == kMaxMatchData) done = true;
@@ -249,6 +373,20 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             ray = -1;
         }
     }
+}
+
+// The lane-refill search (traversal 2) as a kernel of its own per fetch
+// form, so that each form is compiled with the registers it needs.
+template <bool PATHS, bool FAST, int FETCH>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cast_refill(DevScene sc, const float *org3,
+                                                        const float *dir3, const float4 *org4,
+                                                        const float4 *dir4, CountRef n_dev,
+                                                        int64_t n_host, WorkCounter *wc,
+                                                        int *out_id, float *out_t) {
+    const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
+    cast_refill<PATHS, FAST, (FETCH & 1) != 0, (FETCH & 6) == 2, (FETCH & 4) != 0>(
+        sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
+    wave_done(wc);
 }
 
 // Rays given as xyz triples (hpt_cast) or as path state (render).
@@ -262,11 +400,6 @@ __global__ __launch_bounds__(kBlock) void k_cast(DevScene sc, const float *org3,
                                                  WorkCounter *wc, int *out_id, float *out_t,
                                                  int *counts, unsigned long long *totals) {
     const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
-    if (!COUNT && sc.traversal == 2) {
-        cast_refill<PATHS, FAST>(sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
-        wave_done(wc);
-        return;
-    }
     const int lane = threadIdx.x & 63;
     unsigned long long tot[3] = {0, 0, 0};
     for (;;) {
@@ -700,6 +833,25 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
     const bool paths = org4 != nullptr;
     const bool count = counts || totals;
     const bool fast = sc.fast_div != 0;
+    if (!count && sc.traversal == 2) {
+#define HPT_REFILL(P, F, M)                                                                      \
+    k_cast_refill<P, F, M><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, \
+                                                    out_id, out_t)
+#define HPT_REFILL_M(P, F)                                                                       \
+    switch (sc.fetch) {                                                                          \
+    case 1: HPT_REFILL(P, F, 1); break;                                                          \
+    case 2: HPT_REFILL(P, F, 2); break;                                                          \
+    case 3: HPT_REFILL(P, F, 3); break;                                                          \
+    case 4: case 6: HPT_REFILL(P, F, 4); break;                                                  \
+    case 5: case 7: HPT_REFILL(P, F, 5); break;                                                  \
+    default: HPT_REFILL(P, F, 0); break;                                                         \
+    }
+        if (paths) { if (fast) { HPT_REFILL_M(true, true) } else { HPT_REFILL_M(true, false) } }
+        else { if (fast) { HPT_REFILL_M(false, true) } else { HPT_REFILL_M(false, false) } }
+#undef HPT_REFILL_M
+#undef HPT_REFILL
+        return;
+    }
 #define HPT_CAST(C, P, F)                                                                       \
     k_cast<C, P, F><<<grid, kBlock, 0, st>>>(sc, org3, dir3, org4, dir4, n_dev, n_host, wc, out_id, \
                                              out_t, counts, totals)
@@ -752,7 +904,7 @@ void launch_resolve(const ViewParams *views, int view_id, int width, int height,
 
 int occupancy_blocks_cast() {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cast<false, true, true>, kBlock, 0) != hipSuccess) b = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cast_refill<true, true, 0>, kBlock, 0) != hipSuccess) b = 4;
     return b;
 }
 

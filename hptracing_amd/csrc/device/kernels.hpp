@@ -41,11 +41,14 @@ struct DevScene {
     int32_t n_lights;
     const uchar4 *textures;
     int32_t n_textures;
-    int32_t fast_div;  // slab quotients via corrected reciprocals (same results)
+    int32_t fast_div;  // slab quotients via certified reciprocals (same results)
+    int32_t boxes_fast; // every box coordinate is within the certified range
     int32_t traversal; // 0: interleaved per lane; 1: wave phases; 2: phases + lane refill
     int32_t arith_ocl; // dot/cross as ROCm's OpenCL builtins (HPT_ARITH_ROCM_OPENCL)
     int32_t refill;     // traversal 2: idle lanes that trigger a refill
     int32_t node_steps; // traversal 2: node visits per round
+    int32_t fetch;      // traversal 2: bit 0 paired node fetch, bit 1 triangle prefetch,
+                        // bit 2 leaf tests spread over the wave (bit 1 ignored)
 };
 
 struct Paths {

@@ -308,6 +308,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             base.traversal = s->dev.traversal;
             base.refill_threshold = s->dev.refill;
             base.node_steps = s->dev.node_steps;
+            base.fetch = s->dev.fetch;
             base.cast_blocks_per_cu = s->cast_grid / s->n_cus;
             base.shade_blocks_per_cu = s->shade_grid / s->n_cus;
         }
@@ -317,6 +318,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             s->dev.traversal = t.traversal;
             s->dev.refill = t.refill_threshold;
             s->dev.node_steps = t.node_steps;
+            s->dev.fetch = t.fetch;
             s->cast_grid = std::max(1, t.cast_blocks_per_cu) * s->n_cus;
             s->shade_grid = std::max(1, t.shade_blocks_per_cu) * s->n_cus;
         };
@@ -354,7 +356,17 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
                 t.node_steps = steps;
                 consider(t);
             }
-        // Division form and grid size around the best organisation.
+        // Fetch forms, division form and grid size around the best
+        // organisation.
+        if (best.traversal == 2) {
+            const hpt_tuning b = best;
+            for (int f : {0, 1, 2, 3, 4, 5}) {
+                if (f == b.fetch) continue;
+                hpt_tuning t = b;
+                t.fetch = f;
+                consider(t);
+            }
+        }
         {
             hpt_tuning t = best;
             t.fast_division = 1 - t.fast_division;

@@ -146,10 +146,10 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
- * hpt_scene_create (fast_division = 0, traversal = 2, refill_threshold = 16,
- * node_steps = 16, grids = occupancy x CUs). */
+ * hpt_scene_create (fast_division = 1, traversal = 2, refill_threshold = 16,
+ * node_steps = 16, fetch = 4, grids = occupancy x CUs). */
 typedef struct {
-    int32_t fast_division;       /* slab quotients via corrected reciprocals */
+    int32_t fast_division;       /* slab quotients via certified reciprocals */
     int32_t cast_blocks_per_cu;  /* persistent grid of the cast kernel      */
     int32_t shade_blocks_per_cu; /* persistent grid of the shade kernel     */
     int32_t traversal;           /* 0: node and leaf work interleaved per lane,
@@ -158,6 +158,11 @@ typedef struct {
     int32_t refill_threshold;    /* traversal 2: idle lanes (1..64) that make
                                     a wave fetch new rays                   */
     int32_t node_steps;          /* traversal 2: node visits per round      */
+    int32_t fetch;               /* traversal 2: bit 0 fetches nodes in pairs
+                                    (idx, idx + 1), bit 1 fetches triangle
+                                    records one ahead, bit 2 spreads each
+                                    leaf round's triangle tests over the
+                                    wave (then bit 1 is ignored)            */
 } hpt_tuning;
 int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);

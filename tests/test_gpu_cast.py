@@ -12,25 +12,30 @@ from hptracing_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _runner(kd, fast=(0, 2, 16, 16)):
+def _runner(kd, fast=(0, 2, 16, 16, 0)):
     from hptracing_amd.runner import TraceRunner
     r = TraceRunner(kd)
     r.set_tuning(fast_division=fast[0], traversal=fast[1], refill_threshold=fast[2],
-                 node_steps=fast[3])
+                 node_steps=fast[3], fetch=fast[4])
     return r
 
 
-@pytest.fixture(params=[(0, 2, 16, 16), (1, 2, 1, 1), (0, 2, 64, 64), (1, 1, 16, 16),
-                        (0, 1, 16, 16), (1, 0, 16, 16), (0, 0, 16, 16)],
+@pytest.fixture(params=[(0, 2, 16, 16, 0), (1, 2, 1, 1, 0), (0, 2, 64, 64, 0), (1, 1, 16, 16, 0),
+                        (0, 1, 16, 16, 0), (1, 0, 16, 16, 0), (0, 0, 16, 16, 0),
+                        (1, 2, 16, 16, 1), (1, 2, 16, 16, 2), (0, 2, 8, 4, 3), (1, 2, 1, 1, 3),
+                        (1, 2, 16, 16, 4), (0, 2, 1, 1, 4), (1, 2, 64, 4, 5)],
                 ids=["ieee-refill", "rcp-refill-every-step", "ieee-refill-64", "rcp-phases",
-                     "ieee-phases", "rcp-interleaved", "ieee-interleaved"])
+                     "ieee-phases", "rcp-interleaved", "ieee-interleaved", "rcp-refill-pairs",
+                     "rcp-refill-prefetch", "ieee-refill-pairs-prefetch",
+                     "rcp-refill-pairs-prefetch-every-step", "rcp-refill-flat",
+                     "ieee-refill-flat-every-step", "rcp-refill-64-flat-pairs"])
 def fast(request):
-    """(fast_division, traversal, refill_threshold, node_steps) tunings:
-    every one must give the same bits."""
+    """(fast_division, traversal, refill_threshold, node_steps, fetch)
+    tunings: every one must give the same bits."""
     return request.param
 
 
-def _check(kd, org, d, oracle_mod, counted=False, fast=(0, 2, 16, 16)):
+def _check(kd, org, d, oracle_mod, counted=False, fast=(0, 2, 16, 16, 0)):
     import torch
     r = _runner(kd, fast)
     s = oracle_mod.Scene(kd)
@@ -89,3 +94,39 @@ def test_cast_empty(room):
     ids, t = r.cast(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32))
     torch.cuda.synchronize()
     assert ids.numel() == 0
+
+
+def _grazing_rays(kd, n_nodes, seed):
+    """Rays aimed at corners, edge points and face points of KD node boxes,
+    and rays starting on box faces: the slab test's near-ties."""
+    rng = np.random.default_rng(seed)
+    nodes = kd.nodes
+    pick = rng.integers(0, len(nodes), n_nodes)
+    lo = np.stack([nodes["box_start"][pick, k] for k in range(3)], 1).astype(np.float32)
+    hi = np.stack([nodes["box_end"][pick, k] for k in range(3)], 1).astype(np.float32)
+    span = np.maximum(hi - lo, 1e-3).max(1, keepdims=True)
+    corner = np.where(rng.integers(0, 2, (n_nodes, 3)) == 1, hi, lo)
+    edge = corner.copy()
+    ax = rng.integers(0, 3, n_nodes)
+    f = rng.uniform(0, 1, n_nodes).astype(np.float32)
+    edge[np.arange(n_nodes), ax] = (lo + f[:, None] * (hi - lo))[np.arange(n_nodes), ax]
+    face = lo + rng.uniform(0, 1, (n_nodes, 3)).astype(np.float32) * (hi - lo)
+    face[np.arange(n_nodes), ax] = corner[np.arange(n_nodes), ax]
+    orgs, dirs = [], []
+    for target in (corner, edge, face):
+        o = (target + rng.normal(size=(n_nodes, 3)).astype(np.float32) * span * 3).astype(np.float32)
+        orgs.append(o)
+        dirs.append((target - o).astype(np.float32))
+    # starting on a face, leaving along a random or axis-parallel direction
+    d = rng.normal(size=(n_nodes, 3)).astype(np.float32)
+    d[: n_nodes // 4, (ax[: n_nodes // 4] + 1) % 3] = 0.0
+    orgs.append(face.astype(np.float32))
+    dirs.append(d)
+    return np.concatenate(orgs), np.concatenate(dirs)
+
+
+@pytest.mark.parametrize("which", ["room", "soup"])
+def test_cast_grazing_box_edges(room, soup_small, oracle_mod, fast, which):
+    kd = room if which == "room" else soup_small
+    org, d = _grazing_rays(kd, 20000, seed=21)
+    _check(kd, org, d, oracle_mod, fast=fast)
