@@ -193,9 +193,9 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.boxes_fast = boxes_fast ? 1 : 0;
         ds.traversal = 2;
         ds.arith_ocl = 0;
-        ds.refill = 16;
-        ds.node_steps = 16;
-        ds.fetch = 4;
+        ds.refill = 32;
+        ds.node_steps = 32;
+        ds.fetch = 20;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
@@ -250,7 +250,7 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
             t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64 ||
             t->traversal < 0 || t->traversal > 2 || t->refill_threshold < 1 ||
             t->refill_threshold > 64 || t->node_steps < 1 || t->node_steps > 4096 ||
-            t->fetch < 0 || t->fetch > 7)
+            !hpt::valid_fetch(t->fetch))
             fail(HPT_ERR_INVALID, "tuning value out of range");
         std::lock_guard<std::mutex> lock(s->mu);
         s->dev.fast_div = t->fast_division ? 1 : 0;

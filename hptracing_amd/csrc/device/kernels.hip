@@ -166,11 +166,24 @@ __device__ __forceinline__ int wave_scan_max(int v, int lane) {
     return v;
 }
 
-// One leaf round of the FLAT form (see cast_refill).  `cnt` is this lane's
-// pending triangle count (0 if none), `off` the first record of its leaf;
-// best / best_id are folded in place.  All 64 lanes must call it.
-__device__ __forceinline__ void flat_leaf_round(const DevScene &sc, int lane, int cnt, int off,
-                                                V3 sp, V3 d, float &best, int &best_id) {
+// Per-lane queue of pending leaves in LDS (FLAT form): up to Q entries of
+// (first record, triangle count) per lane, lane-major so that a wave's
+// accesses to one slot fall in distinct banks.
+template <int Q>
+struct LeafQueue {
+    int off[Q][kBlock];
+    int cnt[Q][kBlock];
+};
+
+// One leaf round of the FLAT form (see cast_refill).  A lane's pending work
+// is the triangles of its `nq` queued leaves in queue order, `cnt` in all
+// (0 and 0 if none); best / best_id are folded in place.  Queue slots from
+// nq on hold stale or never-written values and are not read.  All 64 lanes
+// must call it.
+template <int Q>
+__device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
+                                                int lane, int nq, int cnt, V3 sp, V3 d,
+                                                float &best, int &best_id) {
     __shared__ float s_r[kBlock];
     __shared__ int s_id[kBlock];
     __shared__ int s_mark[kBlock];
@@ -191,14 +204,25 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, int lane, in
         if (own < 0) own = carry;
         carry = __shfl(own, 63, 64);
         const int o_excl = __shfl(excl, own, 64);
-        const int o_off = __shfl(off, own, 64);
+        const int o_nq = Q > 1 ? __shfl(nq, own, 64) : 1;
         const V3 o_sp = mk(__shfl(sp.x, own, 64), __shfl(sp.y, own, 64), __shfl(sp.z, own, 64));
         const V3 o_d = mk(__shfl(d.x, own, 64), __shfl(d.y, own, 64), __shfl(d.z, own, 64));
         const int j = base + lane;
         float r = -1.f;
         int id = -1;
         if (j < total) {
-            const float4 *rec = sc.tris + 3 * ((int64_t)o_off + (j - o_excl));
+            // Position k of the owner's work: record k of its first queued
+            // leaf, or further along its queue (0 <= k < the owner's count,
+            // the sum of the counts of its o_nq queued leaves).
+            int k = j - o_excl, at = lq.off[0][w + own] + k;
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                if (q < o_nq) {
+                    k -= lq.cnt[q - 1][w + own];
+                    if (k >= 0) at = lq.off[q][w + own] + k;
+                }
+            }
+            const float4 *rec = sc.tris + 3 * (int64_t)at;
             const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
             r = sc.arith_ocl ? tri_hit_ocl(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2))
                              : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
@@ -207,7 +231,7 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, int lane, in
         s_r[w + lane] = r;
         s_id[w + lane] = id;
         __builtin_amdgcn_wave_barrier();
-        // Each owner folds its part of the chunk in leaf order.
+        // Each owner folds its part of the chunk in order.
         if (cnt > 0) {
             const int lo = excl > base ? excl : base;
             const int hi = excl + cnt < base + 64 ? excl + cnt : base + 64;
@@ -227,13 +251,13 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, int lane, in
 // magnitude (a ray skimming the water grid crosses thousands of nodes), so a
 // wave that waits for its slowest lane idles most of its lanes.  Here every
 // lane carries its own ray through a sequence of rounds -- at most
-// `node_steps` node visits until it reaches a non-empty leaf, then that
+// `node_steps` node visits until it holds a non-empty leaf, then that
 // leaf's triangles -- and a lane whose ray is finished takes the next ray
 // from the shared counter as soon as `refill` lanes of the wave are idle.
 // Each ray's leaves are still tested in the order its walk reaches them, so
 // results are those of trace_ray.
 //
-// Two fetch refinements, both leaving every result unchanged:
+// Refinements, none of which changes a result:
 //  PAIRS  nodes are fetched two at a time (idx and idx + 1, 64 contiguous
 //         bytes).  After a leaf, and after an interior node whose box is
 //         hit, the walk moves to idx + 1, so the second node is usually the
@@ -241,32 +265,40 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, int lane, in
 //         only a missed interior node (a jump to its escape) waits.
 //  PREF   a leaf's triangle records are fetched one triangle ahead, so the
 //         loads of the next record are in flight during the current test.
-//  FLAT   the leaf phase is spread over the whole wave: the (ray, triangle)
-//         pairs of every lane's pending leaf are numbered by a wave prefix
-//         sum and tested 64 at a time, one pair per lane, each tester
-//         reading its ray from the owning lane; the owner then folds its
-//         leaf's distances from LDS in the leaf's order, which is the
+//  Q > 0  FLAT form: the leaf phase is spread over the whole wave.  A lane
+//         walks until it has queued Q non-empty leaves (in LDS) or its walk
+//         ends; then the (ray, triangle) pairs of every lane's queue are
+//         numbered by a wave prefix sum and tested 64 at a time, one pair
+//         per lane, each tester reading its ray from the owning lane; the
+//         owner folds its distances from LDS in queue order, which is the
 //         reference's order, so best and ties come out the same.  A leaf
 //         round costs ceil(sum of counts / 64) tests instead of the
-//         largest count.
-template <bool PATHS, bool FAST, bool PAIRS, bool PREF, bool FLAT>
+//         largest count, and lanes keep walking while their queue fills.
+template <bool PATHS, bool FAST, bool PAIRS, bool PREF, int Q>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
                                             const float *dir3, const float4 *org4,
                                             const float4 *dir4, int64_t n, WorkCounter *wc,
                                             int *out_id, float *out_t) {
+    constexpr bool FLAT = Q > 0;
+    constexpr int QN = Q > 0 ? Q : 1;
+    __shared__ LeafQueue<QN> lq;
     const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x;
     const int nn = sc.n_nodes;
     const int threshold = sc.refill < 1 ? 1 : (sc.refill > 64 ? 64 : sc.refill);
     const int node_steps = sc.node_steps < 1 ? 1 : sc.node_steps;
     int64_t ray = -1;
     V3 sp = mk(0.f, 0.f, 0.f), d = sp, inv = sp;
     bool rcp = false, done = true;
-    int idx = 0, leaves = 0, best_id = -1, pcnt = 0, poff = 0;
+    // pending: leaves held for the next leaf round (FLAT: queued in LDS,
+    // pcnt = their triangle total; otherwise pcnt / poff of the one leaf).
+    int idx = 0, leaves = 0, best_id = -1, pcnt = 0, poff = 0, pending = 0;
     float best = -42.f;
     bool exhausted = false;
     // One node visit: box test, then the next index (idx + 1 after a hit
     // interior node or any leaf, the escape after a missed interior node),
-    // and a pending leaf when a non-empty leaf's box is hit.
+    // and a pending leaf when a non-empty leaf's box is hit.  The reference
+    // collects no leaf after its 512th, so the walk ends there.
     auto visit = [&](float4 lo, float4 hi) {
         const int a = __float_as_int(lo.w);
         const int b = __float_as_int(hi.w);
@@ -276,8 +308,16 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             idx = hit ? idx + 1 : a;
         } else {
             if (hit && a > 0) {
-                pcnt = a;
-                poff = b;
+                if (FLAT) {
+                    lq.off[pending][tid] = b;
+                    lq.cnt[pending][tid] = a;
+                    pcnt += a;
+                } else {
+                    pcnt = a;
+                    poff = b;
+                }
+                ++pending;
+                if (leaves + pending == kMaxMatchData) done = true;
             }
             ++idx;
         }
@@ -318,6 +358,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                     best = -42.f;
                     best_id = -1;
                     pcnt = 0;
+                    pending = 0;
                     done = nn <= 0;
                 }
             }
@@ -327,7 +368,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             continue;
         }
         for (int k = 0; k < node_steps; ++k) {
-            const bool walk = ray >= 0 && !done && pcnt == 0;
+            const bool walk = ray >= 0 && !done && pending < QN;
             if (__ballot(walk) == 0) break;
             if (walk) {
                 const float4 lo = sc.nodes[2 * idx];
@@ -337,17 +378,22 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                     const float4 lo1 = sc.nodes[2 * nxt];
                     const float4 hi1 = sc.nodes[2 * nxt + 1];
                     visit(lo, hi);
-                    if (!done && pcnt == 0 && idx == nxt) visit(lo1, hi1);
+                    if (!done && pending < QN && idx == nxt) visit(lo1, hi1);
                 } else {
                     visit(lo, hi);
                 }
             }
         }
         if (FLAT) {
-            flat_leaf_round(sc, lane, ray >= 0 && pcnt > 0 ? pcnt : 0, poff, sp, d, best, best_id);
-            if (ray >= 0 && pcnt > 0) {
+            __builtin_amdgcn_wave_barrier();
+            const bool own = ray >= 0 && pending > 0;
+            flat_leaf_round<QN>(sc, lq, lane, own ? pending : 0, own ? pcnt : 0, sp, d, best,
+                                best_id);
+            if (own) {
+                leaves += pending;
+                if (leaves >= kMaxMatchData) done = true;
                 pcnt = 0;
-                if (++leaves == kMaxMatchData) done = true;
+                pending = 0;
             }
         } else if (ray >= 0 && pcnt > 0) {
             const float4 *rec = sc.tris + 3 * (int64_t)poff;
@@ -365,9 +411,10 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                 for (int k = 0; k < pcnt; ++k) test(rec[3 * k], rec[3 * k + 1], rec[3 * k + 2]);
             }
             pcnt = 0;
+            pending = 0;
             if (++leaves == kMaxMatchData) done = true;
         }
-        if (ray >= 0 && done && pcnt == 0) {
+        if (ray >= 0 && done && pending == 0) {
             out_id[ray] = best_id;
             out_t[ray] = best;
             ray = -1;
@@ -376,16 +423,17 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
 }
 
 // The lane-refill search (traversal 2) as a kernel of its own per fetch
-// form, so that each form is compiled with the registers it needs.
+// form, so that each form is compiled with the registers it needs.  FETCH
+// bits: 1 node pairs, 2 triangle prefetch, 4 FLAT, and with 4 the leaf
+// queue depth 1 << ((FETCH >> 3) & 3).
 template <bool PATHS, bool FAST, int FETCH>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cast_refill(DevScene sc, const float *org3,
-                                                        const float *dir3, const float4 *org4,
-                                                        const float4 *dir4, CountRef n_dev,
-                                                        int64_t n_host, WorkCounter *wc,
-                                                        int *out_id, float *out_t) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cast_refill(
+    DevScene sc, const float *org3, const float *dir3, const float4 *org4, const float4 *dir4,
+    CountRef n_dev, int64_t n_host, WorkCounter *wc, int *out_id, float *out_t) {
     const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
-    cast_refill<PATHS, FAST, (FETCH & 1) != 0, (FETCH & 6) == 2, (FETCH & 4) != 0>(
-        sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
+    constexpr int Q = (FETCH & 4) ? (1 << ((FETCH >> 3) & 3)) : 0;
+    cast_refill<PATHS, FAST, (FETCH & 1) != 0, (FETCH & 6) == 2, Q>(sc, org3, dir3, org4, dir4,
+                                                                   n, wc, out_id, out_t);
     wave_done(wc);
 }
 
@@ -842,8 +890,14 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
     case 1: HPT_REFILL(P, F, 1); break;                                                          \
     case 2: HPT_REFILL(P, F, 2); break;                                                          \
     case 3: HPT_REFILL(P, F, 3); break;                                                          \
-    case 4: case 6: HPT_REFILL(P, F, 4); break;                                                  \
-    case 5: case 7: HPT_REFILL(P, F, 5); break;                                                  \
+    case 4: HPT_REFILL(P, F, 4); break;                                                          \
+    case 5: HPT_REFILL(P, F, 5); break;                                                          \
+    case 12: HPT_REFILL(P, F, 12); break;                                                        \
+    case 13: HPT_REFILL(P, F, 13); break;                                                        \
+    case 20: HPT_REFILL(P, F, 20); break;                                                        \
+    case 21: HPT_REFILL(P, F, 21); break;                                                        \
+    case 28: HPT_REFILL(P, F, 28); break;                                                        \
+    case 29: HPT_REFILL(P, F, 29); break;                                                        \
     default: HPT_REFILL(P, F, 0); break;                                                         \
     }
         if (paths) { if (fast) { HPT_REFILL_M(true, true) } else { HPT_REFILL_M(true, false) } }

@@ -28,5 +28,13 @@ void launch_resolve(const ViewParams *views, int view_id, int width, int height,
                     float brightness, const float *accum, uchar4 *rgba, hipStream_t st);
 int occupancy_blocks_cast();
 int occupancy_blocks_shade();
+// Fetch forms of the lane-refill cast (hpt_tuning.fetch): node pairs (1),
+// triangle prefetch (2), wave-spread leaf rounds (4) with a leaf queue of
+// depth 1, 2, 4 or 8 (bits 3-4 = 0..3; with 4 only, and without 2).
+inline bool valid_fetch(int f) {
+    if (f < 0 || f > 31) return false;
+    if (f & 4) return (f & 2) == 0;
+    return f < 4;
+}
 
 }  // namespace hpt

@@ -360,7 +360,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
         // organisation.
         if (best.traversal == 2) {
             const hpt_tuning b = best;
-            for (int f : {0, 1, 2, 3, 4, 5}) {
+            for (int f : {0, 1, 4, 5, 12, 20, 28}) {
                 if (f == b.fetch) continue;
                 hpt_tuning t = b;
                 t.fetch = f;

@@ -23,12 +23,15 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
 @pytest.fixture(params=[(0, 2, 16, 16, 0), (1, 2, 1, 1, 0), (0, 2, 64, 64, 0), (1, 1, 16, 16, 0),
                         (0, 1, 16, 16, 0), (1, 0, 16, 16, 0), (0, 0, 16, 16, 0),
                         (1, 2, 16, 16, 1), (1, 2, 16, 16, 2), (0, 2, 8, 4, 3), (1, 2, 1, 1, 3),
-                        (1, 2, 16, 16, 4), (0, 2, 1, 1, 4), (1, 2, 64, 4, 5)],
+                        (1, 2, 16, 16, 4), (0, 2, 1, 1, 4), (1, 2, 64, 4, 5), (1, 2, 16, 16, 12),
+                        (0, 2, 1, 1, 20), (1, 2, 32, 8, 29), (1, 2, 32, 16, 20)],
                 ids=["ieee-refill", "rcp-refill-every-step", "ieee-refill-64", "rcp-phases",
                      "ieee-phases", "rcp-interleaved", "ieee-interleaved", "rcp-refill-pairs",
                      "rcp-refill-prefetch", "ieee-refill-pairs-prefetch",
                      "rcp-refill-pairs-prefetch-every-step", "rcp-refill-flat",
-                     "ieee-refill-flat-every-step", "rcp-refill-64-flat-pairs"])
+                     "ieee-refill-flat-every-step", "rcp-refill-64-flat-pairs", "rcp-refill-flat-q2",
+                     "ieee-refill-flat-q4-every-step", "rcp-refill-flat-q8-pairs",
+                     "rcp-refill-32-flat-q4"])
 def fast(request):
     """(fast_division, traversal, refill_threshold, node_steps, fetch)
     tunings: every one must give the same bits."""
