@@ -10,10 +10,16 @@ like the reference's intersect launches: primaries once plus every
 continuation ray of every pass.
 
 Besides the timed render the script reports:
-  roofline      the cast kernel (dominant): algorithmic bytes (32 B per ray,
-                40 B per node box-tested, 4 B per leaf, 56 B per triangle
-                test) over its summed launch time from HIP events on the
-                render stream, against 8 TB/s HBM.
+  roofline      the cast kernel (dominant): algorithmic bytes -- the
+                information content of the reference's search: 32 B per ray
+                (origin and direction in, hit id and t out), 40 B per node
+                box-tested (box and links), 4 B per leaf collected (list
+                entry), 56 B per triangle test (list entry, int4, three
+                vertices) -- over its summed launch time from HIP events on
+                the render stream, against 8 TB/s HBM.  `traffic` is the HBM bytes per
+                launch that rocprofv3 counted (FETCH_SIZE, WRITE_SIZE,
+                scaled by a copy-kernel calibration) for this workload, read
+                from profiles/cast_traffic_full1080p.json when present.
   cpu_baseline  the float32 C restatement of the reference kernels (oracle/)
                 on the host cores, on a bounded sample of the same camera
                 rays at the same spp.
@@ -119,6 +125,8 @@ def main():
         f"launches, shade {tst.shade_ms:.1f} ms in {tst.shade_launches}; "
         f"visits {cst.node_visits}, leaves {cst.leaf_visits}, tests {cst.tri_tests}")
 
+    traffic = measured_traffic(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(kd, view, args.steps, args.cpu_seconds, R)
@@ -165,15 +173,38 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
             "avg_launch_ms": round(per_launch_ms, 3),
             "algorithmic_bytes_per_launch": int(cast_bytes / max(1, tst.cast_launches)),
+            "traffic_source": traffic["source"] if traffic else None,
         },
         "cpu_baseline": cpu,
         "batching": batching,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(args):
+    """HBM bytes per cast launch counted by rocprofv3 on this workload
+    (profiles/cast_traffic_full1080p.json: FETCH_SIZE and WRITE_SIZE per
+    dispatch of the cast kernel, each scaled by what the counter reads for a
+    copy of known size), or None when no measurement of this configuration
+    is recorded."""
+    path = os.path.join(ROOT, "profiles", "cast_traffic_full1080p.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    same = (rec.get("width") == args.width and rec.get("height") == args.height
+            and rec.get("depth") == args.depth and rec.get("water") == args.water
+            and rec.get("supersample") == (not args.no_supersample)
+            and rec.get("profile_steps") == max(1, min(args.profile_steps, args.steps)))
+    if not same:
+        return None
+    return {"bytes_per_launch": int(rec["bytes_per_launch"]),
+            "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, {rec['kernel']}, "
+                      f"{rec['dispatches']} dispatches: {rec['command']}"}
 
 
 def bench_batching(runner, n_views, depth):
@@ -232,6 +263,8 @@ def bench_serving(args):
     log(f"[bench] serving scene: {kd.n_triangles} tris, KD {kd.build_seconds:.1f}s, "
         f"total {time.time() - t0:.1f}s")
     runner = TraceRunner(kd)
+    scene_bytes = int(runner.stats["device_bytes"])
+    log(f"[bench] serving scene on the device: {scene_bytes / 1e9:.2f} GB")
     views = serving.request_views(args.serve_requests, seed=3, depth=args.depth)
     runner.render(views[0])  # warm
     seq_s, seq_rays = serving.one_at_a_time(runner, views)
@@ -248,7 +281,8 @@ def bench_serving(args):
     burst = serving.serve(runner, views, 0.0, seed=11)
     log(f"[bench] served all at once: {burst.wall_s:.2f}s {burst.mrays_s:.1f} Mrays/s, "
         f"p99 {burst.p99_ms:.1f} ms")
-    return {"scene_triangles": kd.n_triangles, "requests": len(views),
+    return {"scene_triangles": kd.n_triangles, "scene_device_bytes": scene_bytes,
+            "requests": len(views),
             "one_at_a_time_s": round(seq_s, 3), "one_at_a_time_mrays_s": round(seq_mrays, 2),
             "served_burst_s": round(burst.wall_s, 3),
             "served_burst_mrays_s": round(burst.mrays_s, 2),
