@@ -295,7 +295,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
         // image shrunk until it has at most max_rays camera rays.
         hpt_view v = *view;
         v.sample = 1;
-        if (max_rays <= 0) max_rays = 1 << 20;
+        if (max_rays <= 0) max_rays = 1 << 23;
         while ((int64_t)v.sample_x * v.supersample_x * v.sample_y * v.supersample_y > max_rays &&
                v.sample_x > 8 && v.sample_y > 8) {
             v.sample_x /= 2;
@@ -333,47 +333,35 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             return best;
         };
         measure(base, true);  // warm caches and code
+        // Coordinate descent from the current setting: each knob in turn
+        // takes the value that gives the shortest cast time with the others
+        // held, two sweeps.  The lane-refill knobs only matter for
+        // traversal 2.
         hpt_tuning best = base;
-        double best_ms = 1e30;
-        auto consider = [&](const hpt_tuning &t) {
-            const double ms = measure(t, true);
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = t;
+        double best_ms = measure(best, true);
+        auto try_knob = [&](auto set, std::initializer_list<int> values) {
+            const hpt_tuning start = best;
+            for (int v : values) {
+                hpt_tuning t = start;
+                set(t, v);
+                if (!valid_fetch(t.fetch)) continue;
+                const double ms = measure(t, true);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                }
             }
         };
-        // Traversal organisation and, for lane refill, its two thresholds.
-        for (int trav = 0; trav <= 1; ++trav) {
-            hpt_tuning t = base;
-            t.traversal = trav;
-            consider(t);
-        }
-        for (int refill : {8, 16, 32, 64})
-            for (int steps : {4, 16, 64}) {
-                hpt_tuning t = base;
-                t.traversal = 2;
-                t.refill_threshold = refill;
-                t.node_steps = steps;
-                consider(t);
+        for (int sweep = 0; sweep < 2; ++sweep) {
+            try_knob([](hpt_tuning &t, int v) { t.traversal = v; }, {0, 1, 2});
+            if (best.traversal == 2) {
+                try_knob([](hpt_tuning &t, int v) { t.fetch = v; }, {0, 1, 4, 5, 12, 20, 28});
+                try_knob([](hpt_tuning &t, int v) { t.refill_threshold = v; },
+                         {8, 16, 24, 32, 48, 64});
+                try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {4, 8, 16, 32, 64});
             }
-        // Fetch forms, division form and grid size around the best
-        // organisation.
-        if (best.traversal == 2) {
-            const hpt_tuning b = best;
-            for (int f : {0, 1, 4, 5, 12, 20, 28}) {
-                if (f == b.fetch) continue;
-                hpt_tuning t = b;
-                t.fetch = f;
-                consider(t);
-            }
-        }
-        {
-            hpt_tuning t = best;
-            t.fast_division = 1 - t.fast_division;
-            consider(t);
-            t = best;
-            t.cast_blocks_per_cu = std::min(64, 2 * std::max(1, best.cast_blocks_per_cu));
-            consider(t);
+            try_knob([](hpt_tuning &t, int v) { t.fast_division = v; }, {0, 1});
+            try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {4, 6, 8});
         }
         const int socc = std::max(1, base.shade_blocks_per_cu);
         double best_shade = 1e30;

@@ -65,8 +65,9 @@ class TraceRunner:
         t = N.Tuning(**cur)
         N.check_device(N.device().hpt_scene_set_tuning(self._h, C.byref(t)))
 
-    def autotune(self, v: N.View, max_rays: int = 1 << 20) -> dict:
-        """Pick the fastest tuning on a reduced one-pass render of `v`."""
+    def autotune(self, v: N.View, max_rays: int = 0) -> dict:
+        """Pick the fastest tuning on one-pass renders of `v` (shrunk to at
+        most max_rays camera rays; 0: the library's default, 2^23)."""
         t = N.Tuning()
         N.check_device(N.device().hpt_autotune(self._h, C.byref(v), max_rays, C.byref(t)))
         return {k: getattr(t, k) for k, _ in N.Tuning._fields_}

@@ -66,3 +66,17 @@ def test_render_deterministic(room):
     a = r.render(v, return_accum=True)[1]
     b = r.render(v, return_accum=True)[1]
     assert np.array_equal(a, b)
+
+
+def test_autotune_keeps_results(room):
+    """The autotuner changes launch forms only: the image is the same bits
+    before and after, and the chosen tuning is the one left on the scene."""
+    from hptracing_amd.runner import TraceRunner, make_view
+    r = TraceRunner(room)
+    v = make_view(**scenes.ROOM_VIEW, width=1.0, height=1.0, sample_x=64, sample_y=64, sample=4,
+                  depth=5, seed=3)
+    _, before = r.render(v, return_accum=True)
+    chosen = r.autotune(v, max_rays=4096)
+    assert chosen == r.get_tuning()
+    _, after = r.render(v, return_accum=True)
+    assert np.array_equal(before.view(np.int32), after.view(np.int32))
