@@ -196,6 +196,7 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.refill = 32;
         ds.node_steps = 32;
         ds.fetch = 20;
+        ds.walk_min = 0;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
@@ -240,6 +241,7 @@ int hpt_scene_get_tuning(const hpt_scene *s, hpt_tuning *out) {
         out->refill_threshold = s->dev.refill;
         out->node_steps = s->dev.node_steps;
         out->fetch = s->dev.fetch;
+        out->walk_min = s->dev.walk_min;
     });
 }
 
@@ -250,7 +252,7 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
             t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64 ||
             t->traversal < 0 || t->traversal > 2 || t->refill_threshold < 1 ||
             t->refill_threshold > 64 || t->node_steps < 1 || t->node_steps > 4096 ||
-            !hpt::valid_fetch(t->fetch))
+            !hpt::valid_fetch(t->fetch) || t->walk_min < 0 || t->walk_min > 64)
             fail(HPT_ERR_INVALID, "tuning value out of range");
         std::lock_guard<std::mutex> lock(s->mu);
         s->dev.fast_div = t->fast_division ? 1 : 0;
@@ -258,6 +260,7 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
         s->dev.refill = t->refill_threshold;
         s->dev.node_steps = t->node_steps;
         s->dev.fetch = t->fetch;
+        s->dev.walk_min = t->walk_min;
         s->cast_grid = t->cast_blocks_per_cu * s->n_cus;
         s->shade_grid = t->shade_blocks_per_cu * s->n_cus;
     });

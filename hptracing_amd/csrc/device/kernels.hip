@@ -184,8 +184,6 @@ template <int Q>
 __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
                                                 int lane, int nq, int cnt, V3 sp, V3 d,
                                                 float &best, int &best_id) {
-    __shared__ float s_r[kBlock];
-    __shared__ int s_id[kBlock];
     __shared__ int s_mark[kBlock];
     const int w = threadIdx.x & ~63;
     const int incl = wave_scan_add(cnt, lane);
@@ -228,20 +226,39 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
                              : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
             id = __float_as_int(r0.w);
         }
-        s_r[w + lane] = r;
-        s_id[w + lane] = id;
-        __builtin_amdgcn_wave_barrier();
-        // Each owner folds its part of the chunk in order.
-        if (cnt > 0) {
-            const int lo = excl > base ? excl : base;
-            const int hi = excl + cnt < base + 64 ? excl + cnt : base + 64;
-            for (int p = lo; p < hi; ++p) {
-                const float rr = s_r[w + p - base];
-                if (rr > 0 && (best < 0 || rr < best)) {
-                    best = rr;
-                    best_id = s_id[w + p - base];
+        // Fold the chunk per owner with a segmented scan over the lanes (a
+        // segment is one owner's run of positions; the chunk start also
+        // starts a segment).  The operator keeps the earlier of two entries
+        // unless only the later has r > 0 or the later's r is smaller -- the
+        // reference's `r > 0 && (best < 0 || r < best)` applied in order --
+        // and is associative, so the value at a segment's last lane is the
+        // sequential fold of that segment.
+        // (Every lane executes each shuffle: a lane that skipped one would
+        // supply no value to its neighbour.)
+        const int own_prev = __shfl_up(own, 1, 64);
+        bool seg = lane == 0 || own != own_prev;
+        for (int step = 1; step < 64; step <<= 1) {
+            const float r_up = __shfl_up(r, step, 64);
+            const int id_up = __shfl_up(id, step, 64);
+            const bool seg_up = __shfl_up(seg ? 1 : 0, step, 64) != 0;
+            if (lane >= step && !seg) {
+                if (r_up > 0 && !(r > 0 && r < r_up)) {
+                    r = r_up;
+                    id = id_up;
                 }
+                seg = seg_up;
             }
+        }
+        // Each owner with positions in this chunk takes its segment's fold,
+        // which follows everything it has folded before.
+        const int last = (excl + cnt < base + 64 ? excl + cnt : base + 64) - 1 - base;
+        const bool mine = cnt > 0 && excl < base + 64 && excl + cnt > base;
+        const int src = mine ? last : lane;
+        const float rr = __shfl(r, src, 64);
+        const int rid = __shfl(id, src, 64);
+        if (mine && rr > 0 && (best < 0 || rr < best)) {
+            best = rr;
+            best_id = rid;
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -254,6 +271,7 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
 // `node_steps` node visits until it holds a non-empty leaf, then that
 // leaf's triangles -- and a lane whose ray is finished takes the next ray
 // from the shared counter as soon as `refill` lanes of the wave are idle.
+// A round's node phase also ends once fewer than `walk_min` lanes can walk.
 // Each ray's leaves are still tested in the order its walk reaches them, so
 // results are those of trace_ray.
 //
@@ -287,6 +305,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
     const int nn = sc.n_nodes;
     const int threshold = sc.refill < 1 ? 1 : (sc.refill > 64 ? 64 : sc.refill);
     const int node_steps = sc.node_steps < 1 ? 1 : sc.node_steps;
+    const int walk_min = sc.walk_min;
     int64_t ray = -1;
     V3 sp = mk(0.f, 0.f, 0.f), d = sp, inv = sp;
     bool rcp = false, done = true;
@@ -367,9 +386,13 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             if (exhausted) break;
             continue;
         }
+        // Node steps until no lane can walk, `node_steps` steps, or (after the
+        // first step, so that every round makes progress) fewer than
+        // `walk_min` lanes walk.
         for (int k = 0; k < node_steps; ++k) {
             const bool walk = ray >= 0 && !done && pending < QN;
-            if (__ballot(walk) == 0) break;
+            const uint64_t wm = __ballot(walk);
+            if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
             if (walk) {
                 const float4 lo = sc.nodes[2 * idx];
                 const float4 hi = sc.nodes[2 * idx + 1];

@@ -47,6 +47,7 @@ struct DevScene {
     int32_t arith_ocl; // dot/cross as ROCm's OpenCL builtins (HPT_ARITH_ROCM_OPENCL)
     int32_t refill;     // traversal 2: idle lanes that trigger a refill
     int32_t node_steps; // traversal 2: node visits per round
+    int32_t walk_min;   // traversal 2: a round's node phase ends below this many walking lanes
     int32_t fetch;      // traversal 2: bit 0 paired node fetch, bit 1 triangle prefetch,
                         // bit 2 leaf tests spread over the wave (not with bit 1),
                         // bits 3-4 (with bit 2) leaf queue depth 1 << value

@@ -309,6 +309,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             base.refill_threshold = s->dev.refill;
             base.node_steps = s->dev.node_steps;
             base.fetch = s->dev.fetch;
+            base.walk_min = s->dev.walk_min;
             base.cast_blocks_per_cu = s->cast_grid / s->n_cus;
             base.shade_blocks_per_cu = s->shade_grid / s->n_cus;
         }
@@ -319,6 +320,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             s->dev.refill = t.refill_threshold;
             s->dev.node_steps = t.node_steps;
             s->dev.fetch = t.fetch;
+            s->dev.walk_min = t.walk_min;
             s->cast_grid = std::max(1, t.cast_blocks_per_cu) * s->n_cus;
             s->shade_grid = std::max(1, t.shade_blocks_per_cu) * s->n_cus;
         };
@@ -359,6 +361,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
                 try_knob([](hpt_tuning &t, int v) { t.refill_threshold = v; },
                          {8, 16, 24, 32, 48, 64});
                 try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {4, 8, 16, 32, 64});
+                try_knob([](hpt_tuning &t, int v) { t.walk_min = v; }, {0, 16, 24, 32, 40});
             }
             try_knob([](hpt_tuning &t, int v) { t.fast_division = v; }, {0, 1});
             try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {4, 6, 8});

@@ -147,7 +147,7 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
  * hpt_scene_create (fast_division = 1, traversal = 2, refill_threshold = 32,
- * node_steps = 32, fetch = 20, grids = occupancy x CUs). */
+ * node_steps = 32, fetch = 20, walk_min = 0, grids = occupancy x CUs). */
 typedef struct {
     int32_t fast_division;       /* slab quotients via certified reciprocals */
     int32_t cast_blocks_per_cu;  /* persistent grid of the cast kernel      */
@@ -165,6 +165,9 @@ typedef struct {
                                     wave (not with bit 1), and with bit 2,
                                     bits 3-4 = q let a lane queue 1 << q
                                     leaves per round                        */
+    int32_t walk_min;            /* traversal 2: a round's node phase ends
+                                    once fewer lanes than this (0..64) can
+                                    walk (0: only when none can)            */
 } hpt_tuning;
 int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);

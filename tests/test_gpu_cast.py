@@ -16,7 +16,7 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
     from hptracing_amd.runner import TraceRunner
     r = TraceRunner(kd)
     r.set_tuning(fast_division=fast[0], traversal=fast[1], refill_threshold=fast[2],
-                 node_steps=fast[3], fetch=fast[4])
+                 node_steps=fast[3], fetch=fast[4], walk_min=fast[5] if len(fast) > 5 else 0)
     return r
 
 
@@ -24,17 +24,19 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                         (0, 1, 16, 16, 0), (1, 0, 16, 16, 0), (0, 0, 16, 16, 0),
                         (1, 2, 16, 16, 1), (1, 2, 16, 16, 2), (0, 2, 8, 4, 3), (1, 2, 1, 1, 3),
                         (1, 2, 16, 16, 4), (0, 2, 1, 1, 4), (1, 2, 64, 4, 5), (1, 2, 16, 16, 12),
-                        (0, 2, 1, 1, 20), (1, 2, 32, 8, 29), (1, 2, 32, 16, 20)],
+                        (0, 2, 1, 1, 20), (1, 2, 32, 8, 29), (1, 2, 32, 16, 20),
+                        (1, 2, 32, 32, 20, 32), (0, 2, 8, 64, 28, 64), (1, 2, 16, 8, 0, 48)],
                 ids=["ieee-refill", "rcp-refill-every-step", "ieee-refill-64", "rcp-phases",
                      "ieee-phases", "rcp-interleaved", "ieee-interleaved", "rcp-refill-pairs",
                      "rcp-refill-prefetch", "ieee-refill-pairs-prefetch",
                      "rcp-refill-pairs-prefetch-every-step", "rcp-refill-flat",
                      "ieee-refill-flat-every-step", "rcp-refill-64-flat-pairs", "rcp-refill-flat-q2",
                      "ieee-refill-flat-q4-every-step", "rcp-refill-flat-q8-pairs",
-                     "rcp-refill-32-flat-q4"])
+                     "rcp-refill-32-flat-q4", "rcp-flat-q4-walk32", "ieee-flat-q8-walk64",
+                     "rcp-refill-walk48"])
 def fast(request):
-    """(fast_division, traversal, refill_threshold, node_steps, fetch)
-    tunings: every one must give the same bits."""
+    """(fast_division, traversal, refill_threshold, node_steps, fetch[,
+    walk_min]) tunings: every one must give the same bits."""
     return request.param
 
 
