@@ -193,7 +193,7 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.boxes_fast = boxes_fast ? 1 : 0;
         ds.traversal = 2;
         ds.arith_ocl = 0;
-        ds.refill = 32;
+        ds.refill = 48;
         ds.node_steps = 32;
         ds.fetch = 20;
         ds.walk_min = 0;

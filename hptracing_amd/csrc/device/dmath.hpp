@@ -98,6 +98,25 @@ __device__ __forceinline__ V3 approx_rcp(V3 d) {
 //
 // The start-inside test reads the same differences: for finite floats (the
 // preconditions) p - s <= 0 exactly when s >= p, with gradual underflow.
+// The certified part alone: the reciprocal form's decision, and in `open`
+// whether it is not certified (the caller then asks box_hit).
+__device__ __forceinline__ bool box_hit_rcp_part(V3 bs, V3 be, V3 sp, V3 y, bool &open) {
+    const float ax0 = bs.x - sp.x, ax1 = be.x - sp.x;
+    const float ay0 = bs.y - sp.y, ay1 = be.y - sp.y;
+    const float az0 = bs.z - sp.z, az1 = be.z - sp.z;
+    const bool inside = fmaxf(fmaxf(ax0, ay0), az0) <= 0.f && fminf(fminf(ax1, ay1), az1) >= 0.f;
+    const float tx0 = ax0 * y.x, tx1 = ax1 * y.x;
+    const float ty0 = ay0 * y.y, ty1 = ay1 * y.y;
+    const float tz0 = az0 * y.z, tz1 = az1 * y.z;
+    const float entry = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    const float exit = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    const float bound = (fabsf(entry) + fabsf(exit)) * 0x1p-16f;
+    const bool sure = fabsf(entry - exit) >= bound || !(bound < INFINITY);
+    const bool ahead = exit > 0.f;
+    open = !inside && ahead && !sure;
+    return inside || (ahead && entry <= exit);
+}
+
 __device__ __forceinline__ bool box_hit_rcp(V3 bs, V3 be, V3 sp, V3 d, V3 y) {
     const float ax0 = bs.x - sp.x, ax1 = be.x - sp.x;
     const float ay0 = bs.y - sp.y, ay1 = be.y - sp.y;

@@ -171,8 +171,8 @@ __device__ __forceinline__ int wave_scan_max(int v, int lane) {
 // accesses to one slot fall in distinct banks.
 template <int Q>
 struct LeafQueue {
-    int off[Q][kBlock];
-    int cnt[Q][kBlock];
+    int off[Q + 1][kBlock];  // slot Q takes the writes of lanes that queue nothing
+    int cnt[Q + 1][kBlock];
 };
 
 // One leaf round of the FLAT form (see cast_refill).  A lane's pending work
@@ -342,6 +342,29 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
         }
         if (idx >= nn) done = true;
     };
+    // The same visit for a wave whose lanes all use the reciprocal slab test
+    // (FLAT form), written without lane branches: the queue write always
+    // happens, into slot QN when nothing is queued, and the rare
+    // uncertified box test is decided by the division form under a
+    // wave-uniform check.
+    auto visit_rcp = [&](float4 lo, float4 hi) {
+        const int a = __float_as_int(lo.w);
+        const int b = __float_as_int(hi.w);
+        bool open;
+        bool hit = box_hit_rcp_part(xyz(lo), xyz(hi), sp, inv, open);
+        if (__ballot(open)) {
+            if (open) hit = box_hit(xyz(lo), xyz(hi), sp, d);
+        }
+        const bool leaf = b >= 0;
+        const bool take = leaf && hit && a > 0;
+        idx = (!leaf && !hit) ? a : idx + 1;
+        const int slot = take ? pending : QN;
+        lq.off[slot][tid] = b;
+        lq.cnt[slot][tid] = a;
+        pcnt += take ? a : 0;
+        pending += take ? 1 : 0;
+        done = done || (take && leaves + pending == kMaxMatchData) || idx >= nn;
+    };
     auto test = [&](float4 r0, float4 r1, float4 r2) {
         const float r = sc.arith_ocl ? tri_hit_ocl(sp, d, xyz(r0), xyz(r1), xyz(r2))
                                      : tri_hit(sp, d, xyz(r0), xyz(r1), xyz(r2));
@@ -386,6 +409,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             if (exhausted) break;
             continue;
         }
+        const bool all_rcp = FLAT && FAST && __ballot(ray >= 0 && !rcp) == 0;
         // Node steps until no lane can walk, `node_steps` steps, or (after the
         // first step, so that every round makes progress) fewer than
         // `walk_min` lanes walk.
@@ -393,7 +417,13 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             const bool walk = ray >= 0 && !done && pending < QN;
             const uint64_t wm = __ballot(walk);
             if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
-            if (walk) {
+            if (all_rcp && !PAIRS) {
+                if (walk) {
+                    const float4 lo = sc.nodes[2 * idx];
+                    const float4 hi = sc.nodes[2 * idx + 1];
+                    visit_rcp(lo, hi);
+                }
+            } else if (walk) {
                 const float4 lo = sc.nodes[2 * idx];
                 const float4 hi = sc.nodes[2 * idx + 1];
                 if (PAIRS) {
