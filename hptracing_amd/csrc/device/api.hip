@@ -280,7 +280,9 @@ static int cast_impl(hpt_scene *s, const float *org, const float *dir, int64_t n
                      float *t, int32_t *counts, void *stream) {
     return guarded([&] {
         if (!s) fail(HPT_ERR_INVALID, "null scene");
-        if (n < 0 || n > INT32_MAX - 64) fail(HPT_ERR_INVALID, "ray count out of range");
+        // The shared ray counter is 32-bit and runs past n by at most two
+        // 64-ray reservations per wave.
+        if (n < 0 || n > ((int64_t)1 << 30)) fail(HPT_ERR_INVALID, "ray count out of range (> 2^30)");
         if (n == 0) return;
         if (!org || !dir || !tri_id || !t) fail(HPT_ERR_INVALID, "null buffer");
         hipStream_t st = stream ? (hipStream_t)stream : s->stream;
