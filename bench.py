@@ -168,7 +168,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cast (KD traversal + triangle intersection)",
+            "kernel": "k_cast_refill (KD traversal + triangle intersection)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

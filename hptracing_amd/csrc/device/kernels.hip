@@ -150,20 +150,62 @@ __device__ __forceinline__ void wave_done(WorkCounter *wc) {
     }
 }
 
-// Wave-wide inclusive prefix sum and running maximum over lanes 0..63.
-__device__ __forceinline__ int wave_scan_add(int v, int lane) {
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(v, off, 64);
-        if (lane >= off) v += y;
-    }
+// Wave-wide inclusive scans over lanes 0..63 on DPP moves (VALU, no LDS
+// round trip): row_shr 1, 2, 4, 8 inside each 16-lane row, then the last
+// lane of each even row into the next row (row_bcast:15), then lane 31 into
+// the upper half (row_bcast:31).  A lane combines only where its source lane
+// exists, so each step extends its range by the range just before it.
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ bool dpp_step_valid(int step, int lane) {
+    return step < 16 ? (lane & 15) >= step : (step == 16 ? (lane & 31) >= 16 : lane >= 32);
+}
+template <class Op>
+__device__ __forceinline__ int wave_scan(int v, int lane, Op op) {
+    int t;
+    t = dpp_mov<0x111>(v); if (dpp_step_valid(1, lane)) v = op(t, v);
+    t = dpp_mov<0x112>(v); if (dpp_step_valid(2, lane)) v = op(t, v);
+    t = dpp_mov<0x114>(v); if (dpp_step_valid(4, lane)) v = op(t, v);
+    t = dpp_mov<0x118>(v); if (dpp_step_valid(8, lane)) v = op(t, v);
+    t = dpp_mov<0x142>(v); if (dpp_step_valid(16, lane)) v = op(t, v);
+    t = dpp_mov<0x143>(v); if (dpp_step_valid(32, lane)) v = op(t, v);
     return v;
 }
+__device__ __forceinline__ int wave_scan_add(int v, int lane) {
+    return wave_scan(v, lane, [](int a, int b) { return a + b; });
+}
 __device__ __forceinline__ int wave_scan_max(int v, int lane) {
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(v, off, 64);
-        if (lane >= off) v = y > v ? y : v;
+    return wave_scan(v, lane, [](int a, int b) { return a > b ? a : b; });
+}
+
+// Segmented fold of (r, id) pairs in lane order: a segment starts at a lane
+// with `seg` set.  The operator keeps the earlier entry unless only the
+// later has r > 0 or the later's r is smaller -- the reference's
+// `r > 0 && (best < 0 || r < best)` applied in order -- and is associative,
+// so after the scan each lane holds the sequential fold of its segment up to
+// itself.
+template <int CTRL>
+__device__ __forceinline__ void seg_step(int step, int lane, float &r, int &id, bool &seg) {
+    const float r_up = __int_as_float(dpp_mov<CTRL>(__float_as_int(r)));
+    const int id_up = dpp_mov<CTRL>(id);
+    const bool seg_up = dpp_mov<CTRL>(seg ? 1 : 0) != 0;
+    if (dpp_step_valid(step, lane) && !seg) {
+        if (r_up > 0 && !(r > 0 && r < r_up)) {
+            r = r_up;
+            id = id_up;
+        }
+        seg = seg_up;
     }
-    return v;
+}
+__device__ __forceinline__ void wave_seg_fold(int lane, float &r, int &id, bool seg) {
+    seg_step<0x111>(1, lane, r, id, seg);
+    seg_step<0x112>(2, lane, r, id, seg);
+    seg_step<0x114>(4, lane, r, id, seg);
+    seg_step<0x118>(8, lane, r, id, seg);
+    seg_step<0x142>(16, lane, r, id, seg);
+    seg_step<0x143>(32, lane, r, id, seg);
 }
 
 // Per-lane queue of pending leaves in LDS (FLAT form): up to Q entries of
@@ -212,12 +254,17 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
             // Position k of the owner's work: record k of its first queued
             // leaf, or further along its queue (0 <= k < the owner's count,
             // the sum of the counts of its o_nq queued leaves).
-            int k = j - o_excl, at = lq.off[0][w + own] + k;
+            // Clamped into the selected leaf, so that a record outside the
+            // owner's leaves is never read.
+            int k = j - o_excl;
+            int c = lq.cnt[0][w + own];
+            int at = lq.off[0][w + own] + min(max(k, 0), c - 1);
 #pragma unroll
             for (int q = 1; q < Q; ++q) {
                 if (q < o_nq) {
-                    k -= lq.cnt[q - 1][w + own];
-                    if (k >= 0) at = lq.off[q][w + own] + k;
+                    k -= c;
+                    c = lq.cnt[q][w + own];
+                    if (k >= 0) at = lq.off[q][w + own] + min(k, c - 1);
                 }
             }
             const float4 *rec = sc.tris + 3 * (int64_t)at;
@@ -226,29 +273,13 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
                              : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
             id = __float_as_int(r0.w);
         }
-        // Fold the chunk per owner with a segmented scan over the lanes (a
-        // segment is one owner's run of positions; the chunk start also
-        // starts a segment).  The operator keeps the earlier of two entries
-        // unless only the later has r > 0 or the later's r is smaller -- the
-        // reference's `r > 0 && (best < 0 || r < best)` applied in order --
-        // and is associative, so the value at a segment's last lane is the
-        // sequential fold of that segment.
-        // (Every lane executes each shuffle: a lane that skipped one would
+        // Fold the chunk per owner: a segment is one owner's run of
+        // positions, and the chunk start also starts one.  (Every lane
+        // executes each cross-lane move: a lane that skipped one would
         // supply no value to its neighbour.)
-        const int own_prev = __shfl_up(own, 1, 64);
-        bool seg = lane == 0 || own != own_prev;
-        for (int step = 1; step < 64; step <<= 1) {
-            const float r_up = __shfl_up(r, step, 64);
-            const int id_up = __shfl_up(id, step, 64);
-            const bool seg_up = __shfl_up(seg ? 1 : 0, step, 64) != 0;
-            if (lane >= step && !seg) {
-                if (r_up > 0 && !(r > 0 && r < r_up)) {
-                    r = r_up;
-                    id = id_up;
-                }
-                seg = seg_up;
-            }
-        }
+        const int own_prev = __shfl(own, lane > 0 ? lane - 1 : 0, 64);
+        const bool seg = lane == 0 || own != own_prev;
+        wave_seg_fold(lane, r, id, seg);
         // Each owner with positions in this chunk takes its segment's fold,
         // which follows everything it has folded before.
         const int last = (excl + cnt < base + 64 ? excl + cnt : base + 64) - 1 - base;
