@@ -93,7 +93,11 @@ struct Workspace {
     DevMem prim_org, prim_dir, prim_thr, prim_id, prim_t, prim_hit, prim_list;
     DevMem a_org, a_dir, a_thr, a_seed, b_org, b_dir, b_thr, b_seed;
     DevMem hit_id, hit_t, accum, rgba, views, segs, live, prim_count, totals, pass, passes;
-    void ensure(int64_t rays, int64_t pixels, int n_passes, int depth, int n_views) {
+    int64_t path_capacity = 0;
+    // rays: camera rays (and accumulator entries per slice); paths: live
+    // paths of one launch (up to kPassSlices per camera ray).
+    void ensure(int64_t rays, int64_t pixels, int n_passes, int depth, int n_views,
+                int64_t paths) {
         if (rays > capacity) {
             const size_t n = (size_t)rays;
             prim_org.reserve(16 * n);
@@ -103,6 +107,11 @@ struct Workspace {
             prim_t.reserve(4 * n);
             prim_hit.reserve(8 * n);
             prim_list.reserve(4 * n);
+            accum.reserve(12 * n * hpt::kPassSlices);
+            capacity = rays;
+        }
+        if (paths > path_capacity) {
+            const size_t n = (size_t)paths;
             a_org.reserve(16 * n);
             a_dir.reserve(16 * n);
             a_thr.reserve(16 * n);
@@ -113,8 +122,7 @@ struct Workspace {
             b_seed.reserve(8 * n);
             hit_id.reserve(4 * n);
             hit_t.reserve(4 * n);
-            accum.reserve(12 * n);
-            capacity = rays;
+            path_capacity = paths;
         }
         rgba.reserve(4 * (size_t)std::max<int64_t>(pixels, 1));
         views.reserve(sizeof(hpt::ViewParams) * (size_t)std::max(n_views, 1));

@@ -661,6 +661,10 @@ __global__ __launch_bounds__(kBlock) void k_primary(const ViewParams *views, int
                 prim_hit[g] = make_int2(id, __float_as_int(hit_t[g]));
                 alive = true;
             }
+            for (int k = 1; k < kPassSlices; ++k) {
+                float *sl = accum + 3 * (g + k * vp.accum_stride);
+                sl[0] = sl[1] = sl[2] = 0.f;
+            }
         }
         int pos;
         const int64_t run = block_compact(alive, prim_count + view_id, pos);
@@ -698,7 +702,9 @@ __global__ __launch_bounds__(kBlock) void k_expand(const ViewParams *views, cons
         const float4 d = prim_dir[g];
         out.org[k] = make_float4(vp.view[0], vp.view[1], vp.view[2], __int_as_float(s.view));
         out.dir[k] = d;
-        out.thr[k] = make_float4(1.f, 1.f, 1.f, __int_as_float(g));
+        // thr.w: the accumulator entry of this path, in its pass's slice.
+        const int64_t entry = g + (int64_t)(s.pass % kPassSlices) * vp.accum_stride;
+        out.thr[k] = make_float4(1.f, 1.f, 1.f, __int_as_float((int)entry));
         out.seed[k] = path_seed(vp.seed, (int64_t)g - vp.ray_base, s.pass);
         out_id[k] = h.x;
         out_t[k] = __int_as_float(h.y);
@@ -950,6 +956,27 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const ViewParams *views, int
     }
 }
 
+// Sum a view's accumulator slices into slice 0, in slice order.
+__global__ __launch_bounds__(kBlock) void k_fold(const ViewParams *views, int view_id,
+                                                 float *accum) {
+    const ViewParams &vp = views[view_id];
+    const int64_t n = (int64_t)vp.sx * vp.sy;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        float *a = accum + 3 * (vp.ray_base + r);
+        float x = a[0], y = a[1], z = a[2];
+        for (int k = 1; k < kPassSlices; ++k) {
+            const float *b = a + 3 * k * vp.accum_stride;
+            x += b[0];
+            y += b[1];
+            z += b[2];
+        }
+        a[0] = x;
+        a[1] = y;
+        a[2] = z;
+    }
+}
+
 // ------------------------------------------------------------ launchers
 static int grid_for(int64_t n, int cap) {
     int64_t g = (n + kBlock - 1) / kBlock;
@@ -1001,6 +1028,10 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
         else { if (fast) HPT_CAST(false, false, true); else HPT_CAST(false, false, false); }
     }
 #undef HPT_CAST
+}
+
+void launch_fold(const ViewParams *views, int view_id, int64_t n, float *accum, hipStream_t st) {
+    k_fold<<<grid_for(n, 8192), kBlock, 0, st>>>(views, view_id, accum);
 }
 
 void launch_camera(const ViewParams *views, int view_id, int64_t n, Paths out, hipStream_t st) {

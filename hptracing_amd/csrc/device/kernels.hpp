@@ -74,7 +74,16 @@ struct ViewParams {
     float width, height;
     uint64_t seed;
     int64_t ray_base;            // first global pixel-sample index of the view
+    int64_t accum_stride;        // camera rays between accumulator slices
 };
+
+// Sample passes run kPassSlices at a time: pass p of a camera ray adds into
+// accumulator slice p % kPassSlices (slices accum_stride camera rays apart),
+// so the paths of one camera ray in flight together never share an entry;
+// each slice takes its passes in increasing order, and the slices are summed
+// in slice order at the end (k_fold).  The sums are the same for a view
+// rendered alone, in a batch or in the server.
+constexpr int kPassSlices = 4;
 
 // A run of primary hits of one view expanded into the path buffer.
 struct Segment {

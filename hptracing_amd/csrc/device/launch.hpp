@@ -13,6 +13,7 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
                  int *out_id, float *out_t, int *counts, unsigned long long *totals, int grid,
                  hipStream_t st);
 void launch_camera(const ViewParams *views, int view_id, int64_t n, Paths out, hipStream_t st);
+void launch_fold(const ViewParams *views, int view_id, int64_t n, float *accum, hipStream_t st);
 void launch_primary(const ViewParams *views, int view_id, int64_t n, const int *hit_id,
                     const float *hit_t, float *accum, int2 *prim_hit, int *prim_list,
                     int *prim_count, hipStream_t st);

@@ -36,10 +36,11 @@ void validate_view(const hpt_view *v) {
     if (v->height < 0 && v->supersample_x != v->supersample_y)
         fail(HPT_ERR_INVALID, "the equirectangular camera needs supersample_x == supersample_y");
     const int64_t R = (int64_t)v->sample_x * v->supersample_x * v->sample_y * v->supersample_y;
-    if (R > INT32_MAX / 2) fail(HPT_ERR_INVALID, "too many camera rays");
+    if (R > INT32_MAX / kPassSlices) fail(HPT_ERR_INVALID, "too many camera rays");
 }
 
-ViewParams params_of(const hpt_view &v, int64_t sx, int64_t sy, int64_t ray_base) {
+ViewParams params_of(const hpt_view &v, int64_t sx, int64_t sy, int64_t ray_base,
+                     int64_t accum_stride) {
     ViewParams vp{};
     for (int k = 0; k < 3; ++k) {
         vp.view[k] = v.view[k];
@@ -62,6 +63,7 @@ ViewParams params_of(const hpt_view &v, int64_t sx, int64_t sy, int64_t ray_base
     vp.height = v.height;
     vp.seed = v.seed;
     vp.ray_base = ray_base;
+    vp.accum_stride = accum_stride;
     return vp;
 }
 
@@ -129,26 +131,29 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
         max_spp = std::max(max_spp, v->sample);
         max_depth = std::max(max_depth, v->depth);
     }
-    if (rays > INT32_MAX / 2) fail(HPT_ERR_INVALID, "too many camera rays in one batch");
+    // Accumulator entries (camera ray + slice * rays) are 32-bit.
+    if (rays > INT32_MAX / kPassSlices) fail(HPT_ERR_INVALID, "too many camera rays in one batch");
+    const int max_groups = (max_spp + kPassSlices - 1) / kPassSlices;
 
     std::lock_guard<std::mutex> lock(s->mu);
     hipStream_t st = s->stream;
     Workspace &ws = s->ws;
-    ws.ensure(rays, pixels, max_spp, max_depth, n);
+    ws.ensure(rays, pixels, max_groups, max_depth, n, 0);
     const bool timed = (flags & HPT_RENDER_TIME_KERNELS) != 0;
     const bool use_graph = !timed && !(flags & HPT_RENDER_NO_GRAPH);
     unsigned long long *totals =
         (flags & HPT_RENDER_COUNT_WORK) ? ws.totals.as<unsigned long long>() : nullptr;
 
     std::vector<ViewParams> vps(n);
-    for (int k = 0; k < n; ++k) vps[k] = params_of(*jobs[k].v, jobs[k].sx, jobs[k].sy, jobs[k].ray_base);
+    for (int k = 0; k < n; ++k)
+        vps[k] = params_of(*jobs[k].v, jobs[k].sx, jobs[k].sy, jobs[k].ray_base, rays);
     ViewParams *d_views = ws.views.as<ViewParams>();
     hip_check(hipMemcpyAsync(d_views, vps.data(), sizeof(ViewParams) * n, hipMemcpyHostToDevice, st),
               "upload views");
     int *prim_count = ws.prim_count.as<int>();
     int *live = ws.live.as<int>();
     hip_check(hipMemsetAsync(prim_count, 0, sizeof(int) * n, st), "memset");
-    hip_check(hipMemsetAsync(live, 0, sizeof(int) * (size_t)max_spp * (max_depth + 1), st), "memset");
+    hip_check(hipMemsetAsync(live, 0, sizeof(int) * (size_t)max_groups * (max_depth + 1), st), "memset");
     hip_check(hipMemsetAsync(ws.pass.p, 0, sizeof(int), st), "memset");
     if (totals) hip_check(hipMemsetAsync(totals, 0, 3 * sizeof(unsigned long long), st), "memset");
 
@@ -168,23 +173,27 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
               "read counts");
     hip_check(hipStreamSynchronize(st), "primary pass");
 
-    // Pass table: pass p covers every view with spp > p.
+    // Launch groups: group q runs passes q*kPassSlices .. (q+1)*kPassSlices-1
+    // of every view with that many samples, all in the same launches.
     std::vector<Segment> segs;
-    std::vector<PassDesc> passes(max_spp);
+    std::vector<PassDesc> passes(max_groups);
     int64_t max_total = 0;
-    for (int p = 0; p < max_spp; ++p) {
+    for (int q = 0; q < max_groups; ++q) {
         PassDesc pd{(int32_t)segs.size(), 0, 0};
         for (int k = 0; k < n; ++k) {
-            if (jobs[k].v->sample <= p || n1[k] == 0) continue;
-            segs.push_back(Segment{k, p, jobs[k].ray_base, n1[k], pd.total});
-            pd.total += n1[k];
-            ++pd.n_segs;
+            if (n1[k] == 0) continue;
+            for (int p = q * kPassSlices; p < (q + 1) * kPassSlices && p < jobs[k].v->sample; ++p) {
+                segs.push_back(Segment{k, p, jobs[k].ray_base, n1[k], pd.total});
+                pd.total += n1[k];
+                ++pd.n_segs;
+            }
         }
-        passes[p] = pd;
+        passes[q] = pd;
         max_total = std::max(max_total, pd.total);
     }
     int live_passes = 0;
-    while (live_passes < max_spp && passes[live_passes].total > 0) ++live_passes;
+    while (live_passes < max_groups && passes[live_passes].total > 0) ++live_passes;
+    ws.ensure(rays, pixels, max_groups, max_depth, n, max_total);
     if (!segs.empty()) {
         ws.segs.reserve(sizeof(Segment) * segs.size());
         hip_check(hipMemcpyAsync(ws.segs.p, segs.data(), sizeof(Segment) * segs.size(),
@@ -220,6 +229,7 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
     }
     hip_check(hipGetLastError(), "render launches");
 
+    for (int k = 0; k < n; ++k) launch_fold(d_views, k, jobs[k].R, ws.accum.as<float>(), st);
     for (int k = 0; k < n; ++k) {
         const hpt_view *v = jobs[k].v;
         if (rgba && rgba[k]) {
@@ -234,7 +244,7 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
                                      12 * (size_t)jobs[k].R, hipMemcpyDeviceToHost, st),
                       "read accum");
     }
-    std::vector<int> lv((size_t)max_spp * (max_depth + 1));
+    std::vector<int> lv((size_t)max_groups * (max_depth + 1));
     hip_check(hipMemcpyAsync(lv.data(), live, sizeof(int) * lv.size(), hipMemcpyDeviceToHost, st),
               "read live counts");
     unsigned long long work[3] = {0, 0, 0};
@@ -247,7 +257,7 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
         stats->primary_rays = rays;
         stats->live_per_depth[0] = rays;
         int64_t cast = rays;
-        for (int p = 0; p < max_spp; ++p)
+        for (int p = 0; p < max_groups; ++p)
             for (int i = 1; i < max_depth; ++i) {
                 const int64_t c = lv[(size_t)p * (max_depth + 1) + i];
                 stats->live_per_depth[i] += c;

@@ -31,9 +31,10 @@ namespace {
 
 constexpr int kMaxSlots = 1024;
 // Device bytes per camera-ray slot of the arena: primary org/dir/thr (48),
-// primary hit id/t (8), cached hit (8), hit list (4), accumulator (12), two
-// path buffers (2 x 56), hit id/t of the live paths (8), RGBA (4).
-constexpr int64_t kBytesPerRay = 48 + 8 + 8 + 4 + 12 + 112 + 8 + 4;
+// primary hit id/t (8), cached hit (8), hit list (4), accumulator slices
+// (12 each), two path buffers (2 x 56), hit id/t of the live paths (8),
+// RGBA (4).
+constexpr int64_t kBytesPerRay = 48 + 8 + 8 + 4 + 12 * kPassSlices + 112 + 8 + 4;
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -167,6 +168,7 @@ void hpt_server::start_requests(const std::vector<Request *> &admitted) {
         vp.height = v.height;
         vp.seed = v.seed;
         vp.ray_base = r->base;
+        vp.accum_stride = capacity;
         hip_check(hipMemcpyAsync(d_views + r->slot, &vp, sizeof(vp), hipMemcpyHostToDevice, st),
                   "upload view");
         hip_check(hipMemsetAsync(prim_count + r->slot, 0, sizeof(int), st), "memset");
@@ -247,6 +249,7 @@ void hpt_server::finish(const std::vector<Request *> &finished) {
     ViewParams *d_views = ws.views.as<ViewParams>();
     for (Request *r : finished) {
         const hpt_view &v = r->v;
+        launch_fold(d_views, r->slot, r->R, ws.accum.as<float>(), st);
         if (r->rgba) {
             uchar4 *img = ws.rgba.as<uchar4>() + r->base;  // pixels <= camera rays
             launch_resolve(d_views, r->slot, v.sample_x, v.sample_y, v.supersample_x,
@@ -320,8 +323,8 @@ int hpt_server_create(hpt_scene *scene, const hpt_server_opts *opts, hpt_server 
         sv->scene = scene;
         sv->capacity = budget / kBytesPerRay;
         if (sv->capacity < 64) fail(HPT_ERR_BUDGET, "device budget too small for any request");
-        if (sv->capacity > INT32_MAX / 2) sv->capacity = INT32_MAX / 2;
-        sv->ws.ensure(sv->capacity, sv->capacity, 1, HPT_MAX_DEPTH, kMaxSlots);
+        if (sv->capacity > INT32_MAX / kPassSlices) sv->capacity = INT32_MAX / kPassSlices;
+        sv->ws.ensure(sv->capacity, sv->capacity, 1, HPT_MAX_DEPTH, kMaxSlots, sv->capacity);
         sv->ws.segs.reserve(sizeof(Segment) * kMaxSlots);
         sv->alloc.init(sv->capacity);
         for (int k = kMaxSlots - 1; k >= 0; --k) sv->free_slots.push_back(k);

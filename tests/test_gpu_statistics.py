@@ -67,7 +67,11 @@ def test_gpu_16k_spp_error(room):
     spp = int(ref["spp"])
     mean = _gpu_mean(room, ref, spp, seed=778)
     se = np.sqrt(2.0) * ref["std"] / np.sqrt(spp)
-    frac, _ = frac_within_4se(mean, ref32["mean"].astype(np.float64), se)
+    # Both images are float32 sums of spp terms, added in different orders
+    # (the GPU sums passes in slices); their rounding differs by up to
+    # spp x 2^-24 relative, which matters on constant-radiance pixels.
+    frac, _ = frac_within_4se(mean, ref32["mean"].astype(np.float64), se,
+                              rel_floor=max(1e-4, spp * 2.0 ** -24))
     rmse64 = float(np.sqrt(np.nanmean((mean - ref["mean"]) ** 2)))
     n_nan = int((~np.isfinite(mean)).any(axis=1).sum())
     print(f"\n[stats] GPU {spp} spp: {frac:.4f} within 4 SE of the float32 reference; "
