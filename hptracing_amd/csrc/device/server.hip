@@ -10,9 +10,9 @@
 //      arena is refused at submit time;
 //   2. for newly admitted requests: camera rays, primary cast, primary-hit
 //      cache (their first intersect with sample_n = spp);
-//   3. one pass over every active request that still has samples to take:
-//      the same expand / cast / shade launches as hpt_render, over the
-//      union of their live paths;
+//   3. the next kPassSlices sample passes of every active request that
+//      still has samples to take: the same expand / cast / shade launches
+//      as hpt_render, over the union of their live paths;
 //   4. requests whose last sample finished are resolved, copied to the
 //      caller's buffers and released; waiters are woken.
 // Requests arriving while others run join at the next macro pass.  Paths
@@ -31,10 +31,10 @@ namespace {
 
 constexpr int kMaxSlots = 1024;
 // Device bytes per camera-ray slot of the arena: primary org/dir/thr (48),
-// primary hit id/t (8), cached hit (8), hit list (4), accumulator slices
-// (12 each), two path buffers (2 x 56), hit id/t of the live paths (8),
-// RGBA (4).
-constexpr int64_t kBytesPerRay = 48 + 8 + 8 + 4 + 12 * kPassSlices + 112 + 8 + 4;
+// primary hit id/t (8), cached hit (8), hit list (4), RGBA (4), and per
+// pass slice an accumulator (12), two path buffers (2 x 56) and hit id/t of
+// the live paths (8).
+constexpr int64_t kBytesPerRay = 48 + 8 + 8 + 4 + 4 + kPassSlices * (12 + 112 + 8);
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -197,13 +197,16 @@ void hpt_server::one_pass(std::vector<Request *> &finished) {
     int depth = 1;
     for (Request *r : active) {
         if (r->next_pass >= r->v.sample) continue;
+        const int end = std::min(r->next_pass + kPassSlices, r->v.sample);
         if (r->n1 > 0) {
-            segs.push_back(Segment{r->slot, r->next_pass, r->base, r->n1, pd.total});
-            pd.total += r->n1;
-            ++pd.n_segs;
+            for (int p = r->next_pass; p < end; ++p) {
+                segs.push_back(Segment{r->slot, p, r->base, r->n1, pd.total});
+                pd.total += r->n1;
+                ++pd.n_segs;
+            }
             depth = std::max(depth, r->v.depth);
         }
-        r->next_pass += 1;
+        r->next_pass = end;
         if (r->next_pass >= r->v.sample) finished.push_back(r);
     }
     if (pd.total > 0) {
@@ -324,8 +327,9 @@ int hpt_server_create(hpt_scene *scene, const hpt_server_opts *opts, hpt_server 
         sv->capacity = budget / kBytesPerRay;
         if (sv->capacity < 64) fail(HPT_ERR_BUDGET, "device budget too small for any request");
         if (sv->capacity > INT32_MAX / kPassSlices) sv->capacity = INT32_MAX / kPassSlices;
-        sv->ws.ensure(sv->capacity, sv->capacity, 1, HPT_MAX_DEPTH, kMaxSlots, sv->capacity);
-        sv->ws.segs.reserve(sizeof(Segment) * kMaxSlots);
+        sv->ws.ensure(sv->capacity, sv->capacity, 1, HPT_MAX_DEPTH, kMaxSlots,
+                      sv->capacity * kPassSlices);
+        sv->ws.segs.reserve(sizeof(Segment) * kMaxSlots * kPassSlices);
         sv->alloc.init(sv->capacity);
         for (int k = kMaxSlots - 1; k >= 0; --k) sv->free_slots.push_back(k);
         hip_check(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking), "hipStreamCreate");

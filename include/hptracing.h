@@ -286,10 +286,10 @@ int hpt_batch_render(hpt_scene *scene, const hpt_view *views, int n_views,
  * counterpart; the reference's GUI re-renders one view at a time on a
  * resident scene, hptracing.cpp:124-130).  A worker thread admits queued
  * requests in arrival order whenever their camera rays fit in the device
- * arena (sized from device_budget_bytes), and runs one pass at a time over
- * every admitted request, so requests that arrive while others render join
- * at the next pass.  A request's result equals its hpt_render result bit
- * for bit. */
+ * arena (sized from device_budget_bytes, about 600 bytes per camera ray),
+ * and runs macro passes of four sample passes over every admitted request,
+ * so requests that arrive while others render join at the next macro pass.
+ * A request's result equals its hpt_render result bit for bit. */
 typedef struct hpt_server hpt_server;
 
 typedef struct {
