@@ -241,12 +241,19 @@ def bench_batching(runner, n_views, depth):
         runner.render(v)
         seq_rays += int(runner.last_stats.rays_cast)
     seq_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for v in views:
+        runner.render(v, pass_at_a_time=True)
+    pat_s = time.perf_counter() - t0
     log(f"[bench] batching {n_views} views: batch {batch_s:.2f}s ({rays / batch_s / 1e6:.1f} "
-        f"Mrays/s), one at a time {seq_s:.2f}s ({seq_rays / seq_s / 1e6:.1f} Mrays/s)")
+        f"Mrays/s), one at a time {seq_s:.2f}s ({seq_rays / seq_s / 1e6:.1f} Mrays/s), one at a "
+        f"time with one sample pass per launch sequence {pat_s:.2f}s")
     return {"views": n_views, "batch_s": round(batch_s, 3), "sequential_s": round(seq_s, 3),
             "batch_mrays_s": round(rays / batch_s / 1e6, 2),
             "sequential_mrays_s": round(seq_rays / seq_s / 1e6, 2),
-            "speedup": round(seq_s / batch_s, 3)}
+            "speedup": round(seq_s / batch_s, 3),
+            "sequential_pass_at_a_time_s": round(pat_s, 3),
+            "speedup_vs_pass_at_a_time": round(pat_s / batch_s, 3)}
 
 
 def bench_serving(args):
@@ -270,6 +277,8 @@ def bench_serving(args):
     seq_s, seq_rays = serving.one_at_a_time(runner, views)
     seq_mrays = seq_rays / seq_s / 1e6
     log(f"[bench] one at a time: {len(views)} requests {seq_s:.2f}s {seq_mrays:.1f} Mrays/s")
+    pat_s, _ = serving.one_at_a_time(runner, views, pass_at_a_time=True)
+    log(f"[bench] one at a time, one sample pass per launch sequence: {pat_s:.2f}s")
     rates = []
     for rate in args.serve_rates:
         r = serving.serve(runner, views, rate, seed=11)
@@ -287,6 +296,8 @@ def bench_serving(args):
             "served_burst_s": round(burst.wall_s, 3),
             "served_burst_mrays_s": round(burst.mrays_s, 2),
             "served_vs_one_at_a_time": round(seq_s / burst.wall_s, 3),
+            "one_at_a_time_pass_at_a_time_s": round(pat_s, 3),
+            "served_vs_one_at_a_time_pass_at_a_time": round(pat_s / burst.wall_s, 3),
             "burst_p50_ms": round(burst.p50_ms, 2), "burst_p99_ms": round(burst.p99_ms, 2),
             "poisson": rates}
 

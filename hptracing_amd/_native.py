@@ -122,6 +122,7 @@ class RenderStats(C.Structure):
 HPT_RENDER_TIME_KERNELS = 1
 HPT_RENDER_COUNT_WORK = 2
 HPT_RENDER_NO_GRAPH = 4
+HPT_RENDER_PASS_AT_A_TIME = 8
 HPT_CAST_COUNTS = 5
 
 

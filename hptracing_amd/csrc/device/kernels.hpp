@@ -83,7 +83,7 @@ struct ViewParams {
 // each slice takes its passes in increasing order, and the slices are summed
 // in slice order at the end (k_fold).  The sums are the same for a view
 // rendered alone, in a batch or in the server.
-constexpr int kPassSlices = 4;
+constexpr int kPassSlices = 8;
 
 // A run of primary hits of one view expanded into the path buffer.
 struct Segment {

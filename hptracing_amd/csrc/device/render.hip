@@ -133,7 +133,10 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
     }
     // Accumulator entries (camera ray + slice * rays) are 32-bit.
     if (rays > INT32_MAX / kPassSlices) fail(HPT_ERR_INVALID, "too many camera rays in one batch");
-    const int max_groups = (max_spp + kPassSlices - 1) / kPassSlices;
+    // Passes per launch sequence: kPassSlices, or one (the reference's
+    // structure) -- the slices, and so the sums, are the same either way.
+    const int group = (flags & HPT_RENDER_PASS_AT_A_TIME) ? 1 : kPassSlices;
+    const int max_groups = (max_spp + group - 1) / group;
 
     std::lock_guard<std::mutex> lock(s->mu);
     hipStream_t st = s->stream;
@@ -173,8 +176,8 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
               "read counts");
     hip_check(hipStreamSynchronize(st), "primary pass");
 
-    // Launch groups: group q runs passes q*kPassSlices .. (q+1)*kPassSlices-1
-    // of every view with that many samples, all in the same launches.
+    // Launch groups: group q runs passes q*group .. (q+1)*group-1 of every
+    // view with that many samples, all in the same launches.
     std::vector<Segment> segs;
     std::vector<PassDesc> passes(max_groups);
     int64_t max_total = 0;
@@ -182,7 +185,7 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
         PassDesc pd{(int32_t)segs.size(), 0, 0};
         for (int k = 0; k < n; ++k) {
             if (n1[k] == 0) continue;
-            for (int p = q * kPassSlices; p < (q + 1) * kPassSlices && p < jobs[k].v->sample; ++p) {
+            for (int p = q * group; p < (q + 1) * group && p < jobs[k].v->sample; ++p) {
                 segs.push_back(Segment{k, p, jobs[k].ray_base, n1[k], pd.total});
                 pd.total += n1[k];
                 ++pd.n_segs;

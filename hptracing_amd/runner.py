@@ -85,13 +85,15 @@ class TraceRunner:
                       disable_diffuse, brightness, seed)
         return self.render(v, return_accum=return_accum)
 
-    def render(self, v: N.View, return_accum=False, time_kernels=False, count_work=False):
+    def render(self, v: N.View, return_accum=False, time_kernels=False, count_work=False,
+               pass_at_a_time=False):
         rgba = np.zeros((v.sample_y, v.sample_x, 4), np.uint8)
         R = v.sample_x * v.supersample_x * v.sample_y * v.supersample_y
         accum = np.zeros((R, 3), np.float32) if return_accum else None
         stats = N.RenderStats()
         flags = (N.HPT_RENDER_TIME_KERNELS if time_kernels else 0) | \
-                (N.HPT_RENDER_COUNT_WORK if count_work else 0)
+                (N.HPT_RENDER_COUNT_WORK if count_work else 0) | \
+                (N.HPT_RENDER_PASS_AT_A_TIME if pass_at_a_time else 0)
         N.check_device(N.device().hpt_render_ex(
             self._h, C.byref(v), rgba.ctypes.data, accum.ctypes.data if return_accum else None,
             C.byref(stats), flags))

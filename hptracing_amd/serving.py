@@ -71,7 +71,7 @@ class ServeResult:
 
 
 def serve(runner: TraceRunner, views: List[N.View], rate: float, seed: int = 1,
-          budget: int = 24 << 30) -> ServeResult:
+          budget: int = 48 << 30) -> ServeResult:
     """Submit `views` with exponential inter-arrival gaps (rate per second;
     rate <= 0 submits them all at once) and wait for every result."""
     rng = np.random.default_rng(seed)
@@ -112,12 +112,13 @@ def serve(runner: TraceRunner, views: List[N.View], rate: float, seed: int = 1,
                        int(st["rays_cast"]), st["rays_cast"] / wall / 1e6, int(st["macro_passes"]))
 
 
-def one_at_a_time(runner: TraceRunner, views: List[N.View]):
-    """Render the same requests sequentially with hpt_render."""
+def one_at_a_time(runner: TraceRunner, views: List[N.View], pass_at_a_time: bool = False):
+    """Render the same requests sequentially with hpt_render (optionally
+    launching one sample pass at a time, the reference's structure)."""
     t0 = time.perf_counter()
     rays = 0
     for v in views:
-        runner.render(v)
+        runner.render(v, pass_at_a_time=pass_at_a_time)
         rays += int(runner.last_stats.rays_cast)
     wall = time.perf_counter() - t0
     return wall, rays

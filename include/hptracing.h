@@ -247,6 +247,10 @@ typedef struct {
 /* Enqueue every pass directly instead of replaying one captured hipGraph
  * of the pass (the graph is also skipped while kernels are timed). */
 #define HPT_RENDER_NO_GRAPH 4u
+/* Launch one sample pass at a time, as TraceRunner::run does, instead of
+ * eight passes per launch sequence.  The image is the same bit for bit
+ * (pass p always accumulates into slice p % 8). */
+#define HPT_RENDER_PASS_AT_A_TIME 8u
 
 /* Render one view.  `rgba` (host, W*H*4 bytes, may be NULL) receives the
  * tone-mapped image exactly as get_image writes it (kernel.cl:615-652;
@@ -286,8 +290,8 @@ int hpt_batch_render(hpt_scene *scene, const hpt_view *views, int n_views,
  * counterpart; the reference's GUI re-renders one view at a time on a
  * resident scene, hptracing.cpp:124-130).  A worker thread admits queued
  * requests in arrival order whenever their camera rays fit in the device
- * arena (sized from device_budget_bytes, about 600 bytes per camera ray),
- * and runs macro passes of four sample passes over every admitted request,
+ * arena (sized from device_budget_bytes, about 1.1 KB per camera ray),
+ * and runs macro passes of eight sample passes over every admitted request,
  * so requests that arrive while others render join at the next macro pass.
  * A request's result equals its hpt_render result bit for bit. */
 typedef struct hpt_server hpt_server;

@@ -76,7 +76,7 @@ def test_server_results_equal_solo_and_overlap(runner):
 
 def test_server_budget_and_timeout(runner):
     from hptracing_amd.runner import RenderServer
-    srv = RenderServer(runner, device_budget_bytes=4 << 20)  # ~7k camera rays
+    srv = RenderServer(runner, device_budget_bytes=8 << 20)  # ~7k camera rays
     with pytest.raises(N.HptError) as e:
         srv.submit(_view(sample_x=200, sample_y=200))
     assert e.value.code == N.HPT_ERR_BUDGET

@@ -80,3 +80,16 @@ def test_autotune_keeps_results(room):
     assert chosen == r.get_tuning()
     _, after = r.render(v, return_accum=True)
     assert np.array_equal(before.view(np.int32), after.view(np.int32))
+
+
+def test_pass_at_a_time_same_bits(room):
+    """Eight passes per launch sequence or one at a time (the reference's
+    launch structure): the accumulator slices make the sums identical."""
+    from hptracing_amd.runner import TraceRunner, make_view
+    r = TraceRunner(room)
+    v = make_view(**scenes.ROOM_VIEW, width=1.0, height=1.0, sample_x=48, sample_y=40, sample=19,
+                  depth=5, seed=11)
+    img_a, acc_a = r.render(v, return_accum=True)
+    img_b, acc_b = r.render(v, return_accum=True, pass_at_a_time=True)
+    assert np.array_equal(acc_a.view(np.int32), acc_b.view(np.int32))
+    assert np.array_equal(img_a, img_b)
