@@ -202,7 +202,10 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
                   "hipDeviceGetAttribute");
         s->n_cus = std::max(1, cus);
-        s->cast_grid = std::max(1, cus * std::max(1, hpt::occupancy_blocks_cast()));
+        // Six resident blocks of the cast per CU (24 waves) run the 1080p
+        // render faster than the eight that fit: fewer rays in flight share
+        // the L1 and L2 better.
+        s->cast_grid = std::max(1, cus * std::max(1, std::min(6, hpt::occupancy_blocks_cast())));
         s->shade_grid = std::max(1, cus * std::max(1, hpt::occupancy_blocks_shade()));
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
         *out = s.release();

@@ -377,7 +377,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
                 try_knob([](hpt_tuning &t, int v) { t.walk_min = v; }, {0, 16, 24, 32, 40});
             }
             try_knob([](hpt_tuning &t, int v) { t.fast_division = v; }, {0, 1});
-            try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {4, 6, 8});
+            try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {4, 5, 6, 7, 8});
         }
         const int socc = std::max(1, base.shade_blocks_per_cu);
         double best_shade = 1e30;
