@@ -195,17 +195,19 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.arith_ocl = 0;
         ds.refill = 48;
         ds.node_steps = 32;
-        ds.fetch = 20;
+        ds.fetch = 124;
         ds.walk_min = 0;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
                   "hipDeviceGetAttribute");
         s->n_cus = std::max(1, cus);
-        // Six resident blocks of the cast per CU (24 waves) run the 1080p
-        // render faster than the eight that fit: fewer rays in flight share
-        // the L1 and L2 better.
-        s->cast_grid = std::max(1, cus * std::max(1, std::min(6, hpt::occupancy_blocks_cast())));
+        // The default cast form (keyed leaf rounds over a queue of 8 leaves,
+        // built for six waves per SIMD) holds 29 KB of LDS per block, so
+        // five blocks (20 waves) fit a CU; they run the 1080p render faster
+        // than the eight blocks of the smaller forms: fewer rays in flight
+        // share the L1 and L2 better.
+        s->cast_grid = std::max(1, cus * std::max(1, std::min(5, hpt::occupancy_blocks_cast())));
         s->shade_grid = std::max(1, cus * std::max(1, hpt::occupancy_blocks_shade()));
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
         *out = s.release();

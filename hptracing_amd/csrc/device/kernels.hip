@@ -295,6 +295,94 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
     }
 }
 
+// Rays of a block's lanes in LDS, written when a lane takes a ray, so that
+// a tester reads its owner's ray with two loads instead of six shuffles.
+struct RaySlots {
+    float4 org[kBlock];
+    float4 dir[kBlock];
+};
+
+// Record index of position k (0 <= k < the owner's count) of the work of the
+// lane whose queue is in column `col`: record k of its first queued leaf, or
+// further along its queue.  Clamped into the selected leaf, so that a record
+// outside the owner's leaves is never read.
+template <int Q>
+__device__ __forceinline__ int queue_record(const LeafQueue<Q> &lq, int col, int nq, int k) {
+    int c = lq.cnt[0][col];
+    int at = lq.off[0][col] + min(max(k, 0), c - 1);
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        if (q < nq) {
+            k -= c;
+            c = lq.cnt[q][col];
+            if (k >= 0) at = lq.off[q][col] + min(k, c - 1);
+        }
+    }
+    return at;
+}
+
+// The leaf round with a keyed fold (KEYED form).  The reference keeps, of a
+// ray's tests in order, the first with the smallest r > 0.  For r > 0
+// (+inf included; NaN never passes r > 0) the order of the float bits is
+// the order of the values, so the key (bits of r) << 32 | position is
+// smallest exactly for that test: every tester with r > 0 takes the 64-bit
+// minimum of its key and the owner's LDS slot, and after the round each
+// owner reads the winner of all its positions at once and combines it with
+// its running best by the reference's rule (the round's tests come after
+// every earlier one, so a tie keeps the earlier best).  The winner's id is
+// read back from its record.  Same contract as flat_leaf_round.
+template <int Q>
+__device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
+                                                 const RaySlots &rs,
+                                                 unsigned long long *key, int lane, int nq,
+                                                 int cnt, float &best, int &best_id) {
+    __shared__ int s_mark[kBlock];
+    const int w = threadIdx.x & ~63;
+    const int incl = wave_scan_add(cnt, lane);
+    const int total = __shfl(incl, 63, 64);
+    const int excl = incl - cnt;
+    key[threadIdx.x] = ~0ULL;
+    int carry = 0;
+    for (int base = 0; base < total; base += 64) {
+        s_mark[w + lane] = -1;
+        __builtin_amdgcn_wave_barrier();
+        if (cnt > 0 && excl >= base && excl < base + 64) s_mark[w + excl - base] = lane;
+        __builtin_amdgcn_wave_barrier();
+        int own = wave_scan_max(s_mark[w + lane], lane);
+        if (own < 0) own = carry;
+        carry = __shfl(own, 63, 64);
+        const int o_excl = __shfl(excl, own, 64);
+        const int o_nq = Q > 1 ? __shfl(nq, own, 64) : 1;
+        const int j = base + lane;
+        if (j < total) {
+            const int k = j - o_excl;
+            const int at = queue_record<Q>(lq, w + own, o_nq, k);
+            const float4 o4 = rs.org[w + own], d4 = rs.dir[w + own];
+            const float4 *rec = sc.tris + 3 * (int64_t)at;
+            const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+            const float r = sc.arith_ocl ? tri_hit_ocl(xyz(o4), xyz(d4), xyz(r0), xyz(r1), xyz(r2))
+                                         : tri_hit(xyz(o4), xyz(d4), xyz(r0), xyz(r1), xyz(r2));
+            if (r > 0.f)
+                atomicMin(key + w + own,
+                          ((unsigned long long)__float_as_uint(r) << 32) | (unsigned)k);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (cnt > 0) {
+        const unsigned long long kv = key[threadIdx.x];
+        if (kv != ~0ULL) {
+            const float r = __uint_as_float((unsigned)(kv >> 32));
+            if (best < 0 || r < best) {
+                best = r;
+                const int at = queue_record<Q>(lq, threadIdx.x, nq, (int)(unsigned)kv);
+                best_id = __float_as_int(sc.tris[3 * (int64_t)at].w);
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // The search with per-lane refill.  Per-ray costs differ by orders of
 // magnitude (a ray skimming the water grid crosses thousands of nodes), so a
 // wave that waits for its slowest lane idles most of its lanes.  Here every
@@ -323,7 +411,9 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
 //         reference's order, so best and ties come out the same.  A leaf
 //         round costs ceil(sum of counts / 64) tests instead of the
 //         largest count, and lanes keep walking while their queue fills.
-template <bool PATHS, bool FAST, bool PAIRS, bool PREF, int Q>
+//  KEYED  (FLAT only) the leaf round folds through 64-bit keys in LDS
+//         (keyed_leaf_round) and testers read rays from LDS.
+template <bool PATHS, bool FAST, bool PAIRS, bool PREF, int Q, bool KEYED>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
                                             const float *dir3, const float4 *org4,
                                             const float4 *dir4, int64_t n, WorkCounter *wc,
@@ -331,6 +421,8 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
     constexpr bool FLAT = Q > 0;
     constexpr int QN = Q > 0 ? Q : 1;
     __shared__ LeafQueue<QN> lq;
+    __shared__ RaySlots rs;
+    __shared__ unsigned long long s_key[kBlock];
     const int lane = threadIdx.x & 63;
     const int tid = threadIdx.x;
     const int nn = sc.n_nodes;
@@ -426,6 +518,10 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                     }
                     rcp = FAST && sc.boxes_fast && fast_box_ray(sp, d);
                     inv = approx_rcp(d);
+                    if (KEYED) {
+                        rs.org[tid] = make_float4(sp.x, sp.y, sp.z, 0.f);
+                        rs.dir[tid] = make_float4(d.x, d.y, d.z, 0.f);
+                    }
                     idx = 0;
                     leaves = 0;
                     best = -42.f;
@@ -471,8 +567,12 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
         if (FLAT) {
             __builtin_amdgcn_wave_barrier();
             const bool own = ray >= 0 && pending > 0;
-            flat_leaf_round<QN>(sc, lq, lane, own ? pending : 0, own ? pcnt : 0, sp, d, best,
-                                best_id);
+            if (KEYED)
+                keyed_leaf_round<QN>(sc, lq, rs, s_key, lane, own ? pending : 0,
+                                     own ? pcnt : 0, best, best_id);
+            else
+                flat_leaf_round<QN>(sc, lq, lane, own ? pending : 0, own ? pcnt : 0, sp, d,
+                                    best, best_id);
             if (own) {
                 leaves += pending;
                 if (leaves >= kMaxMatchData) done = true;
@@ -509,15 +609,25 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
 // The lane-refill search (traversal 2) as a kernel of its own per fetch
 // form, so that each form is compiled with the registers it needs.  FETCH
 // bits: 1 node pairs, 2 triangle prefetch, 4 FLAT, and with 4 the leaf
-// queue depth 1 << ((FETCH >> 3) & 3).
+// queue depth 1 << ((FETCH >> 3) & 3); bit 32 compiles the form for six
+// waves per SIMD (80 VGPRs) instead of eight (64); bit 64 selects the keyed
+// leaf-round fold.
+// The keyed forms hold 21 KB (queue of 4) or 29 KB (queue of 8) of LDS per
+// block, so at most 7 or 5 blocks fit in a CU's 160 KB.
+constexpr int cast_waves_per_eu(int fetch) {
+    const int regs = (fetch & 32) ? 6 : 8;
+    const int lds = (fetch & 64) ? (((fetch >> 3) & 3) == 3 ? 5 : 7) : 8;
+    return regs < lds ? regs : lds;
+}
 template <bool PATHS, bool FAST, int FETCH>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cast_refill(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+    cast_waves_per_eu(FETCH), cast_waves_per_eu(FETCH)))) void k_cast_refill(
     DevScene sc, const float *org3, const float *dir3, const float4 *org4, const float4 *dir4,
     CountRef n_dev, int64_t n_host, WorkCounter *wc, int *out_id, float *out_t) {
     const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
     constexpr int Q = (FETCH & 4) ? (1 << ((FETCH >> 3) & 3)) : 0;
-    cast_refill<PATHS, FAST, (FETCH & 1) != 0, (FETCH & 6) == 2, Q>(sc, org3, dir3, org4, dir4,
-                                                                   n, wc, out_id, out_t);
+    cast_refill<PATHS, FAST, (FETCH & 1) != 0, (FETCH & 6) == 2, Q, (FETCH & 64) != 0>(
+        sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
     wave_done(wc);
 }
 
@@ -1009,6 +1119,12 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
     case 21: HPT_REFILL(P, F, 21); break;                                                        \
     case 28: HPT_REFILL(P, F, 28); break;                                                        \
     case 29: HPT_REFILL(P, F, 29); break;                                                        \
+    case 52: HPT_REFILL(P, F, 52); break;                                                        \
+    case 60: HPT_REFILL(P, F, 60); break;                                                        \
+    case 84: HPT_REFILL(P, F, 84); break;                                                        \
+    case 92: HPT_REFILL(P, F, 92); break;                                                        \
+    case 116: HPT_REFILL(P, F, 116); break;                                                      \
+    case 124: HPT_REFILL(P, F, 124); break;                                                      \
     default: HPT_REFILL(P, F, 0); break;                                                         \
     }
         if (paths) { if (fast) { HPT_REFILL_M(true, true) } else { HPT_REFILL_M(true, false) } }

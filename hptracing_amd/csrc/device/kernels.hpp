@@ -50,7 +50,8 @@ struct DevScene {
     int32_t walk_min;   // traversal 2: a round's node phase ends below this many walking lanes
     int32_t fetch;      // traversal 2: bit 0 paired node fetch, bit 1 triangle prefetch,
                         // bit 2 leaf tests spread over the wave (not with bit 1),
-                        // bits 3-4 (with bit 2) leaf queue depth 1 << value
+                        // bits 3-4 (with bit 2) leaf queue depth 1 << value,
+                        // bit 5 six-wave register budget, bit 6 keyed leaf fold
 };
 
 struct Paths {
@@ -83,7 +84,7 @@ struct ViewParams {
 // each slice takes its passes in increasing order, and the slices are summed
 // in slice order at the end (k_fold).  The sums are the same for a view
 // rendered alone, in a batch or in the server.
-constexpr int kPassSlices = 8;
+constexpr int kPassSlices = 16;
 
 // A run of primary hits of one view expanded into the path buffer.
 struct Segment {

@@ -370,14 +370,15 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
         for (int sweep = 0; sweep < 2; ++sweep) {
             try_knob([](hpt_tuning &t, int v) { t.traversal = v; }, {0, 1, 2});
             if (best.traversal == 2) {
-                try_knob([](hpt_tuning &t, int v) { t.fetch = v; }, {0, 1, 4, 5, 12, 20, 28});
+                try_knob([](hpt_tuning &t, int v) { t.fetch = v; },
+                         {0, 1, 4, 5, 12, 20, 28, 52, 60, 84, 92, 116, 124});
                 try_knob([](hpt_tuning &t, int v) { t.refill_threshold = v; },
                          {8, 16, 24, 32, 48, 64});
                 try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {4, 8, 16, 32, 64});
                 try_knob([](hpt_tuning &t, int v) { t.walk_min = v; }, {0, 16, 24, 32, 40});
             }
             try_knob([](hpt_tuning &t, int v) { t.fast_division = v; }, {0, 1});
-            try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {4, 5, 6, 7, 8});
+            try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {3, 4, 5, 6, 7, 8});
         }
         const int socc = std::max(1, base.shade_blocks_per_cu);
         double best_shade = 1e30;

@@ -10,7 +10,7 @@
 //      arena is refused at submit time;
 //   2. for newly admitted requests: camera rays, primary cast, primary-hit
 //      cache (their first intersect with sample_n = spp);
-//   3. the next kPassSlices sample passes of every active request that
+//   3. the next kServerPasses sample passes of every active request that
 //      still has samples to take: the same expand / cast / shade launches
 //      as hpt_render, over the union of their live paths;
 //   4. requests whose last sample finished are resolved, copied to the
@@ -30,11 +30,18 @@ using namespace hpt;
 namespace {
 
 constexpr int kMaxSlots = 1024;
+// Sample passes per macro pass.  Fewer than a render's launch group
+// (kPassSlices), so that a request arriving while others render waits less
+// for the next admission; pass p still adds into slice p % kPassSlices, so
+// results are the same bits as a render's.
+constexpr int kServerPasses = 8;
+static_assert(kServerPasses <= kPassSlices, "passes in flight must use distinct slices");
 // Device bytes per camera-ray slot of the arena: primary org/dir/thr (48),
-// primary hit id/t (8), cached hit (8), hit list (4), RGBA (4), and per
-// pass slice an accumulator (12), two path buffers (2 x 56) and hit id/t of
-// the live paths (8).
-constexpr int64_t kBytesPerRay = 48 + 8 + 8 + 4 + 4 + kPassSlices * (12 + 112 + 8);
+// primary hit id/t (8), cached hit (8), hit list (4), RGBA (4), an
+// accumulator per pass slice (12 each), and per pass of a macro pass two
+// path buffers (2 x 56) and hit id/t of the live paths (8).
+constexpr int64_t kBytesPerRay =
+    48 + 8 + 8 + 4 + 4 + kPassSlices * 12 + kServerPasses * (112 + 8);
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -197,7 +204,7 @@ void hpt_server::one_pass(std::vector<Request *> &finished) {
     int depth = 1;
     for (Request *r : active) {
         if (r->next_pass >= r->v.sample) continue;
-        const int end = std::min(r->next_pass + kPassSlices, r->v.sample);
+        const int end = std::min(r->next_pass + kServerPasses, r->v.sample);
         if (r->n1 > 0) {
             for (int p = r->next_pass; p < end; ++p) {
                 segs.push_back(Segment{r->slot, p, r->base, r->n1, pd.total});
@@ -328,8 +335,8 @@ int hpt_server_create(hpt_scene *scene, const hpt_server_opts *opts, hpt_server 
         if (sv->capacity < 64) fail(HPT_ERR_BUDGET, "device budget too small for any request");
         if (sv->capacity > INT32_MAX / kPassSlices) sv->capacity = INT32_MAX / kPassSlices;
         sv->ws.ensure(sv->capacity, sv->capacity, 1, HPT_MAX_DEPTH, kMaxSlots,
-                      sv->capacity * kPassSlices);
-        sv->ws.segs.reserve(sizeof(Segment) * kMaxSlots * kPassSlices);
+                      sv->capacity * kServerPasses);
+        sv->ws.segs.reserve(sizeof(Segment) * kMaxSlots * kServerPasses);
         sv->alloc.init(sv->capacity);
         for (int k = kMaxSlots - 1; k >= 0; --k) sv->free_slots.push_back(k);
         hip_check(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking), "hipStreamCreate");

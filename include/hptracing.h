@@ -147,7 +147,7 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
  * hpt_scene_create (fast_division = 1, traversal = 2, refill_threshold = 48,
- * node_steps = 32, fetch = 20, walk_min = 0, cast grid = min(6, occupancy)
+ * node_steps = 32, fetch = 124, walk_min = 0, cast grid = min(5, occupancy)
  * x CUs, shade grid = occupancy x CUs). */
 typedef struct {
     int32_t fast_division;       /* slab quotients via certified reciprocals */
@@ -165,7 +165,13 @@ typedef struct {
                                     leaf round's triangle tests over the
                                     wave (not with bit 1), and with bit 2,
                                     bits 3-4 = q let a lane queue 1 << q
-                                    leaves per round                        */
+                                    leaves per round; with a queue of 4 or
+                                    8 and without bit 0, bit 5 builds the
+                                    kernel for six waves per SIMD and bit 6
+                                    folds each leaf round through 64-bit
+                                    (distance, position) keys in LDS.  The
+                                    compiled forms: 0-5, 12, 13, 20, 21,
+                                    28, 29, 52, 60, 84, 92, 116, 124        */
     int32_t walk_min;            /* traversal 2: a round's node phase ends
                                     once fewer lanes than this (0..64) can
                                     walk (0: only when none can)            */
@@ -249,8 +255,8 @@ typedef struct {
  * of the pass (the graph is also skipped while kernels are timed). */
 #define HPT_RENDER_NO_GRAPH 4u
 /* Launch one sample pass at a time, as TraceRunner::run does, instead of
- * eight passes per launch sequence.  The image is the same bit for bit
- * (pass p always accumulates into slice p % 8). */
+ * sixteen passes per launch sequence.  The image is the same bit for bit
+ * (pass p always accumulates into slice p % 16). */
 #define HPT_RENDER_PASS_AT_A_TIME 8u
 
 /* Render one view.  `rgba` (host, W*H*4 bytes, may be NULL) receives the
@@ -291,7 +297,7 @@ int hpt_batch_render(hpt_scene *scene, const hpt_view *views, int n_views,
  * counterpart; the reference's GUI re-renders one view at a time on a
  * resident scene, hptracing.cpp:124-130).  A worker thread admits queued
  * requests in arrival order whenever their camera rays fit in the device
- * arena (sized from device_budget_bytes, about 1.1 KB per camera ray),
+ * arena (sized from device_budget_bytes, about 1.2 KB per camera ray),
  * and runs macro passes of eight sample passes over every admitted request,
  * so requests that arrive while others render join at the next macro pass.
  * A request's result equals its hpt_render result bit for bit. */

@@ -25,7 +25,9 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                         (1, 2, 16, 16, 1), (1, 2, 16, 16, 2), (0, 2, 8, 4, 3), (1, 2, 1, 1, 3),
                         (1, 2, 16, 16, 4), (0, 2, 1, 1, 4), (1, 2, 64, 4, 5), (1, 2, 16, 16, 12),
                         (0, 2, 1, 1, 20), (1, 2, 32, 8, 29), (1, 2, 32, 16, 20),
-                        (1, 2, 32, 32, 20, 32), (0, 2, 8, 64, 28, 64), (1, 2, 16, 8, 0, 48)],
+                        (1, 2, 32, 32, 20, 32), (0, 2, 8, 64, 28, 64), (1, 2, 16, 8, 0, 48),
+                        (1, 2, 48, 32, 52), (0, 2, 8, 64, 60), (1, 2, 48, 32, 84),
+                        (0, 2, 1, 1, 116), (1, 2, 16, 8, 124), (1, 2, 32, 16, 92, 16)],
                 ids=["ieee-refill", "rcp-refill-every-step", "ieee-refill-64", "rcp-phases",
                      "ieee-phases", "rcp-interleaved", "ieee-interleaved", "rcp-refill-pairs",
                      "rcp-refill-prefetch", "ieee-refill-pairs-prefetch",
@@ -33,7 +35,9 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                      "ieee-refill-flat-every-step", "rcp-refill-64-flat-pairs", "rcp-refill-flat-q2",
                      "ieee-refill-flat-q4-every-step", "rcp-refill-flat-q8-pairs",
                      "rcp-refill-32-flat-q4", "rcp-flat-q4-walk32", "ieee-flat-q8-walk64",
-                     "rcp-refill-walk48"])
+                     "rcp-refill-walk48", "rcp-flat-q4-wide", "ieee-flat-q8-wide",
+                     "rcp-flat-q4-keyed", "ieee-flat-q4-wide-keyed-every-step",
+                     "rcp-flat-q8-wide-keyed", "rcp-flat-q8-keyed-walk16"])
 def fast(request):
     """(fast_division, traversal, refill_threshold, node_steps, fetch[,
     walk_min]) tunings: every one must give the same bits."""
@@ -135,3 +139,16 @@ def test_cast_grazing_box_edges(room, soup_small, oracle_mod, fast, which):
     kd = room if which == "room" else soup_small
     org, d = _grazing_rays(kd, 20000, seed=21)
     _check(kd, org, d, oracle_mod, fast=fast)
+
+
+def test_cast_exact_ties(oracle_mod, fast):
+    """Every triangle twice, under two ids: each hit is an exact tie in t,
+    which the reference breaks by test order (leaf order, then list order;
+    kernel.cl:170-185), not by id."""
+    from hptracing_amd.scene import KDTree, material
+    pts, mesh = scenes.soup(4000, seed=8)
+    mesh2 = np.concatenate([mesh, mesh])
+    kd = KDTree.from_arrays(pts, mesh2, np.array([material(kd=(0.5, 0.5, 0.5))]))
+    org, d = scenes.random_rays(30000, (0, 0, 0), (1, 1, 1), seed=14)
+    ids, _ = _check(kd, org, d, oracle_mod, fast=fast)
+    assert (ids >= 0).mean() > 0.05
