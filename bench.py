@@ -23,6 +23,14 @@ Besides the timed render the script reports:
   cpu_baseline  the float32 C restatement of the reference kernels (oracle/)
                 on the host cores, on a bounded sample of the same camera
                 rays at the same spp.
+  cast_only     the ray-cast stage in isolation (hpt_cast): 2^22 rays with
+                origins uniform in the scene box and directions uniform on
+                the sphere through a seeded triangle soup (centres uniform
+                in the unit cube, edges of log-uniform length in
+                [1e-3, 1e-1]) of 200 000 triangles, the largest the
+                reference's KD builder fits in memory; device time from HIP
+                events, and its algorithmic GB/s from the same batch's
+                counted walk.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -63,6 +71,8 @@ def main():
     ap.add_argument("--batch-views", type=int, default=64, help="0 skips the batching leg")
     ap.add_argument("--serve-requests", type=int, default=40, help="0 skips the serving leg")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[2.0, 8.0])
+    ap.add_argument("--cast-tris", type=int, default=200_000,
+                    help="triangles of the cast-only soup (0 skips the cast-only leg)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -140,6 +150,8 @@ def main():
         del runner
         serving = bench_serving(args)
 
+    cast_only = bench_cast_only(args.cast_tris) if args.cast_tris > 0 else None
+
     out = {
         "metric": "Mrays/s (all bounces)",
         "value": round(value, 3),
@@ -181,6 +193,7 @@ def main():
         "cpu_baseline": cpu,
         "batching": batching,
         "serving": serving,
+        "cast_only": cast_only,
     }
     print(json.dumps(out), flush=True)
 
@@ -300,6 +313,56 @@ def bench_serving(args):
             "served_vs_one_at_a_time_pass_at_a_time": round(pat_s / burst.wall_s, 3),
             "burst_p50_ms": round(burst.p50_ms, 2), "burst_p99_ms": round(burst.p99_ms, 2),
             "poisson": rates}
+
+
+def bench_cast_only(n_tris):
+    """hpt_cast alone on the ray-cast workload: Mrays/s over the median of
+    five launches, and algorithmic bytes (the render's per-unit figures) from
+    the counted walk of the same rays."""
+    import numpy as np
+    import torch
+
+    from hptracing_amd import scenes
+    from hptracing_amd.runner import TraceRunner
+    from hptracing_amd.scene import KDTree, material
+
+    t0 = time.time()
+    pts, mesh = scenes.soup(n_tris, seed=1)
+    kd = KDTree.from_arrays(pts, mesh, np.array([material(kd=(0.5, 0.5, 0.5))]))
+    p = kd.points[:, :3]
+    n = 1 << 22
+    org, d = scenes.random_rays(n, p.min(0), p.max(0), seed=41)
+    runner = TraceRunner(kd)
+    org_t = torch.as_tensor(org).cuda()
+    dir_t = torch.as_tensor(d).cuda()
+    runner.cast(org_t, dir_t)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(5):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        runner.cast(org_t, dir_t)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    _, _, cnt = runner.cast(org_t, dir_t, counted=True)
+    torch.cuda.synchronize()
+    c = cnt.sum(dim=0, dtype=torch.int64).cpu().numpy()
+    algo = 32 * n + 40 * int(c[0]) + 4 * int(c[1]) + 56 * int(c[2])
+    gbs = algo / (ms * 1e-3) / 1e9
+    log(f"[bench] cast only: {kd.n_triangles} tris ({len(kd.leaf_data)} leaf ints), {n} rays, "
+        f"{ms:.2f} ms, {n / ms / 1e3:.1f} Mrays/s, {algo / n:.0f} algorithmic B/ray, "
+        f"{gbs:.0f} GB/s; setup {time.time() - t0:.1f}s")
+    return {"workload": f"hpt_cast, {n} rays through a {kd.n_triangles}-triangle soup",
+            "triangles": kd.n_triangles, "rays": n, "ms": round(ms, 3),
+            "mrays_s": round(n / ms / 1e3, 2),
+            "node_visits_per_ray": round(int(c[0]) / n, 1),
+            "leaves_per_ray": round(int(c[1]) / n, 1),
+            "tri_tests_per_ray": round(int(c[2]) / n, 1),
+            "algorithmic_bytes": algo, "achieved_gbs": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def cpu_baseline(kd, view, spp, target_s, R):

@@ -290,7 +290,9 @@ static int cast_impl(hpt_scene *s, const float *org, const float *dir, int64_t n
         if (n < 0 || n > ((int64_t)1 << 30)) fail(HPT_ERR_INVALID, "ray count out of range (> 2^30)");
         if (n == 0) return;
         if (!org || !dir || !tri_id || !t) fail(HPT_ERR_INVALID, "null buffer");
-        hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+        // NULL is HIP's null stream, so a cast follows whatever the caller
+        // enqueued there to produce its rays.
+        hipStream_t st = (hipStream_t)stream;
         hpt::launch_cast(s->dev, org, dir, nullptr, nullptr, hpt::CountRef{}, n, s->counter(),
                          tri_id, t, counts, nullptr, s->cast_grid, st);
         hip_check(hipGetLastError(), "k_cast launch");

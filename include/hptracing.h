@@ -196,7 +196,8 @@ int hpt_scene_set_arithmetic(hpt_scene *scene, int mode);
  * miss) and its distance t (-42 on a miss, the kernel's initial value).  The
  * reference keeps only the triangle's int4 (kernel.cl:191); the index is the
  * position of that int4 in the mesh.  n <= 2^30 per call.  Asynchronous on
- * `stream`. */
+ * `stream` (a hipStream_t; NULL is the null stream, ordered with the work
+ * of every blocking stream). */
 int hpt_cast(hpt_scene *scene, const float *org, const float *dir, int64_t n,
              int32_t *tri_id, float *t, void *stream);
 

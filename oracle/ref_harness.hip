@@ -55,10 +55,13 @@ const char *ref_last_error(void) { return g_err.c_str(); }
 
 // points: 4 floats per point; mesh: int4; nodes: 48-byte KDTreeNodeHeader;
 // org/dir: xyz triples.  geometry: 4 ints per ray, t: 1 float per ray.
-int ref_kdtree_intersect(const char *co_path, const float *points, int64_t n_points,
-                         const int32_t *mesh, int64_t n_tris, const void *nodes, int64_t n_nodes,
-                         const int32_t *leaf, int64_t n_leaf, const float *org, const float *dir,
-                         int32_t n, int32_t *geometry, float *t) {
+// kernel_ms (may be NULL): device time of the kdtree_intersect launch, from
+// HIP events around it.
+int ref_kdtree_intersect_timed(const char *co_path, const float *points, int64_t n_points,
+                               const int32_t *mesh, int64_t n_tris, const void *nodes,
+                               int64_t n_nodes, const int32_t *leaf, int64_t n_leaf,
+                               const float *org, const float *dir, int32_t n, int32_t *geometry,
+                               float *t, float *kernel_ms) {
     std::vector<void *> keep;
     hipModule_t mod = nullptr;
     int rc = -1;
@@ -99,10 +102,21 @@ int ref_kdtree_intersect(const char *co_path, const float *points, int64_t n_poi
         void *args[] = {&d_sizes, &d_data, &d_s0, &d_s1, &d_points, &d_mesh, &d_leaf,
                         &d_nodes, &kd_node_size, &d_result, &bg, &sample_n};
         const unsigned groups = (unsigned)(n / 256 + 1);
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (kernel_ms && (!ok(hipEventCreate(&ev[0]), "hipEventCreate") ||
+                          !ok(hipEventCreate(&ev[1]), "hipEventCreate")))
+            break;
+        if (kernel_ms && !ok(hipEventRecord(ev[0], nullptr), "hipEventRecord")) break;
         if (!ok(hipModuleLaunchKernel(fn, groups, 1, 1, 256, 1, 1, 0, nullptr, args, nullptr),
                 "hipModuleLaunchKernel"))
             break;
+        if (kernel_ms && !ok(hipEventRecord(ev[1], nullptr), "hipEventRecord")) break;
         if (!ok(hipDeviceSynchronize(), "kdtree_intersect")) break;
+        if (kernel_ms) {
+            if (!ok(hipEventElapsedTime(kernel_ms, ev[0], ev[1]), "hipEventElapsedTime")) break;
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+        }
         if (!ok(hipMemcpy(data.data(), d_data, sizeof(Unit) * n, hipMemcpyDeviceToHost), "download"))
             break;
         for (int32_t i = 0; i < n; ++i) {
@@ -114,6 +128,14 @@ int ref_kdtree_intersect(const char *co_path, const float *points, int64_t n_poi
     for (void *p : keep) (void)hipFree(p);
     if (mod) (void)hipModuleUnload(mod);
     return rc;
+}
+
+int ref_kdtree_intersect(const char *co_path, const float *points, int64_t n_points,
+                         const int32_t *mesh, int64_t n_tris, const void *nodes, int64_t n_nodes,
+                         const int32_t *leaf, int64_t n_leaf, const float *org, const float *dir,
+                         int32_t n, int32_t *geometry, float *t) {
+    return ref_kdtree_intersect_timed(co_path, points, n_points, mesh, n_tris, nodes, n_nodes,
+                                      leaf, n_leaf, org, dir, n, geometry, t, nullptr);
 }
 
 }  // extern "C"

@@ -68,6 +68,16 @@ def soup_small():
 
 
 @pytest.fixture(scope="session")
+def soup_200k():
+    """The ray-cast workload's soup at the largest size the reference's KD
+    builder fits in memory (200 000 triangles, 57 M leaf entries)."""
+    from hptracing_amd import scenes
+    from hptracing_amd.scene import KDTree, material
+    pts, mesh = scenes.soup(200_000, seed=1)
+    return KDTree.from_arrays(pts, mesh, np.array([material(kd=(0.5, 0.5, 0.5))]))
+
+
+@pytest.fixture(scope="session")
 def oracle_mod():
     import oracle
     return oracle

@@ -46,13 +46,6 @@ def mt_float32(org, d, pa, pb, pc):
     return np.where(ok, t, f(-1)), u, v
 
 
-@pytest.fixture(scope="module")
-def soup_200k():
-    from hptracing_amd.scene import KDTree, material
-    pts, mesh = scenes.soup(200_000, seed=1)
-    return KDTree.from_arrays(pts, mesh, np.array([material(kd=(0.5, 0.5, 0.5))]))
-
-
 def test_cast_full_size_properties(soup_200k, oracle_mod):
     import torch
     from hptracing_amd.runner import TraceRunner

@@ -26,24 +26,29 @@ needs_ref = pytest.mark.skipif(not (os.path.exists(CO) and os.path.exists(LIB)),
                                reason="reference kernels not built (oracle/build_ref.sh)")
 
 
-def ref_cast(kd, org, d):
+def ref_cast(kd, org, d, timed=False):
+    """The reference kernel's geometry int4 and t per ray (and, with
+    `timed`, its launch's device time in ms)."""
     lib = C.CDLL(LIB)
     vp = C.c_void_p
-    lib.ref_kdtree_intersect.argtypes = [C.c_char_p, vp, C.c_int64, vp, C.c_int64, vp, C.c_int64,
-                                         vp, C.c_int64, vp, vp, C.c_int32, vp, vp]
+    lib.ref_kdtree_intersect_timed.argtypes = [C.c_char_p, vp, C.c_int64, vp, C.c_int64, vp,
+                                               C.c_int64, vp, C.c_int64, vp, vp, C.c_int32, vp,
+                                               vp, vp]
     lib.ref_last_error.restype = C.c_char_p
     org = np.ascontiguousarray(org, np.float32)
     d = np.ascontiguousarray(d, np.float32)
     n = len(org)
     geo = np.empty((n, 4), np.int32)
     t = np.empty(n, np.float32)
-    rc = lib.ref_kdtree_intersect(CO.encode(), kd.points.ctypes.data, len(kd.points),
-                                  kd.geometries.ctypes.data, len(kd.geometries),
-                                  kd.nodes.ctypes.data, len(kd.nodes), kd.leaf_data.ctypes.data,
-                                  len(kd.leaf_data), org.ctypes.data, d.ctypes.data, n,
-                                  geo.ctypes.data, t.ctypes.data)
+    ms = C.c_float(0.0)
+    rc = lib.ref_kdtree_intersect_timed(CO.encode(), kd.points.ctypes.data, len(kd.points),
+                                        kd.geometries.ctypes.data, len(kd.geometries),
+                                        kd.nodes.ctypes.data, len(kd.nodes),
+                                        kd.leaf_data.ctypes.data, len(kd.leaf_data),
+                                        org.ctypes.data, d.ctypes.data, n, geo.ctypes.data,
+                                        t.ctypes.data, C.byref(ms))
     assert rc == 0, lib.ref_last_error()
-    return geo, t
+    return (geo, t, ms.value) if timed else (geo, t)
 
 
 def compare(kd, org, d, rocm_opencl=False):
@@ -102,3 +107,41 @@ def test_cast_literal_arithmetic_same_hits(which, request):
           f"quantiles 50/90/99% {np.percentile(rel, [50, 90, 99])}")
     assert same.all()
     assert within >= (0.99 if which == "room" else 0.90)
+
+
+@needs_ref
+def test_full_size_batch_against_reference_kernel(soup_200k):
+    """The ray-cast workload at full batch size -- 2^22 rays, origins uniform
+    in the scene's box, directions uniform on the sphere -- through the
+    200 000-triangle soup: every hit triangle and every distance equals the
+    reference kernel's (ROCm OpenCL arithmetic).  Both kernels' device times
+    on the same batch are printed (the reference kernel as compiled for the
+    parity comparison: -O3, no contraction, correctly rounded division)."""
+    import torch
+    from hptracing_amd.runner import TraceRunner
+    kd = soup_200k
+    pts = kd.points[:, :3]
+    org, d = scenes.random_rays(1 << 22, pts.min(0), pts.max(0), seed=41)
+    r = TraceRunner(kd)
+    r.set_arithmetic(True)
+    org_t = torch.as_tensor(org).cuda()
+    dir_t = torch.as_tensor(d).cuda()
+    r.cast(org_t, dir_t)  # warm
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ids, t = r.cast(org_t, dir_t)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    ids, t = ids.cpu().numpy(), t.cpu().numpy()
+    ref_geo, ref_t, ref_ms = ref_cast(kd, org, d, timed=True)
+    ours_geo = np.where(ids[:, None] >= 0, kd.geometries[np.maximum(ids, 0)], -1)
+    n = len(ids)
+    print(f"\n[ref-opencl] 2^22 rays, 200k-triangle soup: reference kdtree_intersect "
+          f"{ref_ms:.2f} ms ({n / ref_ms / 1e3:.1f} Mrays/s), hpt_cast {best:.2f} ms "
+          f"({n / best / 1e3:.1f} Mrays/s), {ref_ms / best:.1f}x; hits {(ids >= 0).sum()}")
+    assert (ours_geo == ref_geo).all()
+    assert np.array_equal(t.view(np.int32), ref_t.view(np.int32))
