@@ -539,28 +539,37 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
         const bool all_rcp = FLAT && FAST && __ballot(ray >= 0 && !rcp) == 0;
         // Node steps until no lane can walk, `node_steps` steps, or (after the
         // first step, so that every round makes progress) fewer than
-        // `walk_min` lanes walk.
-        for (int k = 0; k < node_steps; ++k) {
-            const bool walk = ray >= 0 && !done && pending < QN;
-            const uint64_t wm = __ballot(walk);
-            if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
-            if (all_rcp && !PAIRS) {
+        // `walk_min` lanes walk.  The loop is written once per visit form
+        // (the form is fixed for the round), so that the loop-carried
+        // walk state needs no merge of two code paths on every step.
+        if (all_rcp && !PAIRS) {
+            for (int k = 0; k < node_steps; ++k) {
+                const bool walk = ray >= 0 && !done && pending < QN;
+                const uint64_t wm = __ballot(walk);
+                if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
                 if (walk) {
                     const float4 lo = sc.nodes[2 * idx];
                     const float4 hi = sc.nodes[2 * idx + 1];
                     visit_rcp(lo, hi);
                 }
-            } else if (walk) {
-                const float4 lo = sc.nodes[2 * idx];
-                const float4 hi = sc.nodes[2 * idx + 1];
-                if (PAIRS) {
-                    const int nxt = idx + 1 < nn ? idx + 1 : idx;
-                    const float4 lo1 = sc.nodes[2 * nxt];
-                    const float4 hi1 = sc.nodes[2 * nxt + 1];
-                    visit(lo, hi);
-                    if (!done && pending < QN && idx == nxt) visit(lo1, hi1);
-                } else {
-                    visit(lo, hi);
+            }
+        } else {
+            for (int k = 0; k < node_steps; ++k) {
+                const bool walk = ray >= 0 && !done && pending < QN;
+                const uint64_t wm = __ballot(walk);
+                if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
+                if (walk) {
+                    const float4 lo = sc.nodes[2 * idx];
+                    const float4 hi = sc.nodes[2 * idx + 1];
+                    if (PAIRS) {
+                        const int nxt = idx + 1 < nn ? idx + 1 : idx;
+                        const float4 lo1 = sc.nodes[2 * nxt];
+                        const float4 hi1 = sc.nodes[2 * nxt + 1];
+                        visit(lo, hi);
+                        if (!done && pending < QN && idx == nxt) visit(lo1, hi1);
+                    } else {
+                        visit(lo, hi);
+                    }
                 }
             }
         }
