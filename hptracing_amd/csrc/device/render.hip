@@ -134,8 +134,15 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
     // Accumulator entries (camera ray + slice * rays) are 32-bit.
     if (rays > INT32_MAX / kPassSlices) fail(HPT_ERR_INVALID, "too many camera rays in one batch");
     // Passes per launch sequence: kPassSlices, or one (the reference's
-    // structure) -- the slices, and so the sums, are the same either way.
-    const int group = (flags & HPT_RENDER_PASS_AT_A_TIME) ? 1 : kPassSlices;
+    // structure), and fewer when the path buffers of kPassSlices passes
+    // (two ping-pong buffers of 56 B and a hit id/t of 8 B per path) would
+    // pass 48 GiB -- an 8K image with 2x2 supersampling, say.  Pass p adds
+    // into slice p % kPassSlices in any grouping, so the sums are the same.
+    constexpr int64_t kPathBytes = 2 * 56 + 8;
+    constexpr int64_t kPathBudget = (int64_t)48 << 30;
+    int group = (flags & HPT_RENDER_PASS_AT_A_TIME) ? 1 : kPassSlices;
+    group = (int)std::max<int64_t>(
+        1, std::min<int64_t>(group, kPathBudget / std::max<int64_t>(1, rays * kPathBytes)));
     const int max_groups = (max_spp + group - 1) / group;
 
     std::lock_guard<std::mutex> lock(s->mu);

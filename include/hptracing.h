@@ -256,7 +256,8 @@ typedef struct {
  * of the pass (the graph is also skipped while kernels are timed). */
 #define HPT_RENDER_NO_GRAPH 4u
 /* Launch one sample pass at a time, as TraceRunner::run does, instead of
- * sixteen passes per launch sequence.  The image is the same bit for bit
+ * sixteen passes per launch sequence (fewer when the path buffers of
+ * sixteen passes would pass 48 GiB).  The image is the same bit for bit
  * (pass p always accumulates into slice p % 16). */
 #define HPT_RENDER_PASS_AT_A_TIME 8u
 
