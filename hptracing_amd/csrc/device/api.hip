@@ -194,19 +194,18 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.traversal = 2;
         ds.arith_ocl = 0;
         ds.refill = 48;
-        ds.node_steps = 32;
-        ds.fetch = 124;
+        ds.node_steps = 24;
+        ds.fetch = 29;
         ds.walk_min = 0;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
                   "hipDeviceGetAttribute");
         s->n_cus = std::max(1, cus);
-        // The default cast form (keyed leaf rounds over a queue of 8 leaves,
-        // built for six waves per SIMD) holds 29 KB of LDS per block, so
-        // five blocks (20 waves) fit a CU; they run the 1080p render faster
-        // than the eight blocks of the smaller forms: fewer rays in flight
-        // share the L1 and L2 better.
+        // Five resident blocks of the cast per CU (20 waves) run the 1080p
+        // render faster than the eight that fit with the default form (node
+        // pairs, a queue of 8 leaves): fewer rays in flight share the L1 and
+        // L2 better.
         s->cast_grid = std::max(1, cus * std::max(1, std::min(5, hpt::occupancy_blocks_cast())));
         s->shade_grid = std::max(1, cus * std::max(1, hpt::occupancy_blocks_shade()));
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");

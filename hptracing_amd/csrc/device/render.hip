@@ -378,10 +378,10 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             try_knob([](hpt_tuning &t, int v) { t.traversal = v; }, {0, 1, 2});
             if (best.traversal == 2) {
                 try_knob([](hpt_tuning &t, int v) { t.fetch = v; },
-                         {0, 1, 4, 5, 12, 20, 28, 52, 60, 84, 92, 116, 124});
+                         {0, 1, 4, 5, 12, 13, 20, 21, 28, 29, 52, 60, 84, 92, 116, 124});
                 try_knob([](hpt_tuning &t, int v) { t.refill_threshold = v; },
                          {8, 16, 24, 32, 48, 64});
-                try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {4, 8, 16, 32, 64});
+                try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {8, 16, 24, 32, 48, 64});
                 try_knob([](hpt_tuning &t, int v) { t.walk_min = v; }, {0, 16, 24, 32, 40});
             }
             try_knob([](hpt_tuning &t, int v) { t.fast_division = v; }, {0, 1});

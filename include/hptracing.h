@@ -147,7 +147,7 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
  * hpt_scene_create (fast_division = 1, traversal = 2, refill_threshold = 48,
- * node_steps = 32, fetch = 124, walk_min = 0, cast grid = min(5, occupancy)
+ * node_steps = 24, fetch = 29, walk_min = 0, cast grid = min(5, occupancy)
  * x CUs, shade grid = occupancy x CUs). */
 typedef struct {
     int32_t fast_division;       /* slab quotients via certified reciprocals */
