@@ -124,6 +124,7 @@ HPT_RENDER_COUNT_WORK = 2
 HPT_RENDER_NO_GRAPH = 4
 HPT_RENDER_PASS_AT_A_TIME = 8
 HPT_CAST_COUNTS = 5
+HPT_SERVE_PROGRESSIVE = 1
 
 
 class Tuning(C.Structure):
@@ -216,6 +217,10 @@ def device() -> C.CDLL:
         lib.hpt_scene_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         lib.hpt_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
         lib.hpt_server_submit.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(C.c_int64)]
+        lib.hpt_server_submit_ex.argtypes = [vp, C.POINTER(View), vp, vp, C.c_uint,
+                                             C.POINTER(C.c_int64)]
+        lib.hpt_server_peek.argtypes = [vp, C.c_int64, C.c_int32, C.c_double, vp,
+                                        C.POINTER(C.c_int32)]
         lib.hpt_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(RequestInfo)]
         lib.hpt_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
         lib.hpt_server_destroy.argtypes = [vp]

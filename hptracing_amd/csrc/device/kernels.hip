@@ -1048,8 +1048,8 @@ __device__ __forceinline__ unsigned char unorm8(float f) {
 
 __global__ __launch_bounds__(kBlock) void k_resolve(const ViewParams *views, int view_id,
                                                     int width, int height, int ssx, int ssy,
-                                                    float brightness, const float *accum,
-                                                    uchar4 *rgba) {
+                                                    int samples, float brightness,
+                                                    const float *accum, uchar4 *rgba) {
     const ViewParams &vp = views[view_id];
     const int64_t n = (int64_t)width * height;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
@@ -1064,7 +1064,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const ViewParams *views, int
                 py += accum[index * 3 + 1];
                 pz += accum[index * 3 + 2];
             }
-        const float nss = (float)(ssx * ssy), sp = (float)vp.sample;
+        const float nss = (float)(ssx * ssy), sp = (float)samples;
         px /= nss; py /= nss; pz /= nss;
         px /= sp; py /= sp; pz /= sp;
         px *= brightness; py *= brightness; pz *= brightness;
@@ -1075,14 +1075,16 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const ViewParams *views, int
     }
 }
 
-// Sum a view's accumulator slices into slice 0, in slice order.
+// Sum a view's accumulator slices in slice order into `out` (same camera-
+// ray indexing as slice 0; `out` may be `accum` itself: each thread reads
+// all slices of its entry before writing it).
 __global__ __launch_bounds__(kBlock) void k_fold(const ViewParams *views, int view_id,
-                                                 float *accum) {
+                                                 const float *accum, float *out) {
     const ViewParams &vp = views[view_id];
     const int64_t n = (int64_t)vp.sx * vp.sy;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += (int64_t)gridDim.x * blockDim.x) {
-        float *a = accum + 3 * (vp.ray_base + r);
+        const float *a = accum + 3 * (vp.ray_base + r);
         float x = a[0], y = a[1], z = a[2];
         for (int k = 1; k < kPassSlices; ++k) {
             const float *b = a + 3 * k * vp.accum_stride;
@@ -1090,9 +1092,31 @@ __global__ __launch_bounds__(kBlock) void k_fold(const ViewParams *views, int vi
             y += b[1];
             z += b[2];
         }
-        a[0] = x;
-        a[1] = y;
-        a[2] = z;
+        float *o = out + 3 * (vp.ray_base + r);
+        o[0] = x;
+        o[1] = y;
+        o[2] = z;
+    }
+}
+
+// Background of the camera rays whose primary ray missed, for an image of
+// `samples` passes: what k_primary writes for a render of that many
+// samples (slice 0 = samples x background; the paths never touch these
+// entries).  Lets a progressive request resolve after any pass count.
+__global__ __launch_bounds__(kBlock) void k_misses(const ViewParams *views, int view_id,
+                                                   const int *hit_id, int samples,
+                                                   float *accum) {
+    const ViewParams &vp = views[view_id];
+    const int64_t n = (int64_t)vp.sx * vp.sy;
+    const float s = (float)samples;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = vp.ray_base + r;
+        if (hit_id[g] >= 0) continue;
+        float *acc = accum + 3 * g;
+        acc[0] = s * (vp.background[0] * 1.f);
+        acc[1] = s * (vp.background[1] * 1.f);
+        acc[2] = s * (vp.background[2] * 1.f);
     }
 }
 
@@ -1155,8 +1179,14 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
 #undef HPT_CAST
 }
 
-void launch_fold(const ViewParams *views, int view_id, int64_t n, float *accum, hipStream_t st) {
-    k_fold<<<grid_for(n, 8192), kBlock, 0, st>>>(views, view_id, accum);
+void launch_fold(const ViewParams *views, int view_id, int64_t n, const float *accum, float *out,
+                 hipStream_t st) {
+    k_fold<<<grid_for(n, 8192), kBlock, 0, st>>>(views, view_id, accum, out);
+}
+
+void launch_misses(const ViewParams *views, int view_id, int64_t n, const int *hit_id,
+                   int samples, float *accum, hipStream_t st) {
+    k_misses<<<grid_for(n, 8192), kBlock, 0, st>>>(views, view_id, hit_id, samples, accum);
 }
 
 void launch_camera(const ViewParams *views, int view_id, int64_t n, Paths out, hipStream_t st) {
@@ -1191,9 +1221,10 @@ __global__ void k_next_pass(int *pass) { *pass += 1; }
 void launch_next_pass(int *pass, hipStream_t st) { k_next_pass<<<1, 1, 0, st>>>(pass); }
 
 void launch_resolve(const ViewParams *views, int view_id, int width, int height, int ssx, int ssy,
-                    float brightness, const float *accum, uchar4 *rgba, hipStream_t st) {
+                    int samples, float brightness, const float *accum, uchar4 *rgba,
+                    hipStream_t st) {
     k_resolve<<<grid_for((int64_t)width * height, 8192), kBlock, 0, st>>>(
-        views, view_id, width, height, ssx, ssy, brightness, accum, rgba);
+        views, view_id, width, height, ssx, ssy, samples, brightness, accum, rgba);
 }
 
 int occupancy_blocks_cast() {

@@ -13,7 +13,10 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
                  int *out_id, float *out_t, int *counts, unsigned long long *totals, int grid,
                  hipStream_t st);
 void launch_camera(const ViewParams *views, int view_id, int64_t n, Paths out, hipStream_t st);
-void launch_fold(const ViewParams *views, int view_id, int64_t n, float *accum, hipStream_t st);
+void launch_fold(const ViewParams *views, int view_id, int64_t n, const float *accum, float *out,
+                 hipStream_t st);
+void launch_misses(const ViewParams *views, int view_id, int64_t n, const int *hit_id,
+                   int samples, float *accum, hipStream_t st);
 void launch_primary(const ViewParams *views, int view_id, int64_t n, const int *hit_id,
                     const float *hit_t, float *accum, int2 *prim_hit, int *prim_list,
                     int *prim_count, hipStream_t st);
@@ -26,7 +29,8 @@ void launch_shade(const DevScene &sc, const ViewParams *views, int depth_index, 
                   float *accum, int grid, hipStream_t st);
 void launch_next_pass(int *pass, hipStream_t st);
 void launch_resolve(const ViewParams *views, int view_id, int width, int height, int ssx, int ssy,
-                    float brightness, const float *accum, uchar4 *rgba, hipStream_t st);
+                    int samples, float brightness, const float *accum, uchar4 *rgba,
+                    hipStream_t st);
 int occupancy_blocks_cast();
 int occupancy_blocks_shade();
 // Fetch forms of the lane-refill cast (hpt_tuning.fetch): node pairs (1),

@@ -239,13 +239,14 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
     }
     hip_check(hipGetLastError(), "render launches");
 
-    for (int k = 0; k < n; ++k) launch_fold(d_views, k, jobs[k].R, ws.accum.as<float>(), st);
+    for (int k = 0; k < n; ++k)
+        launch_fold(d_views, k, jobs[k].R, ws.accum.as<float>(), ws.accum.as<float>(), st);
     for (int k = 0; k < n; ++k) {
         const hpt_view *v = jobs[k].v;
         if (rgba && rgba[k]) {
             uchar4 *img = ws.rgba.as<uchar4>() + jobs[k].pix_base;
             launch_resolve(d_views, k, v->sample_x, v->sample_y, v->supersample_x, v->supersample_y,
-                           v->brightness, ws.accum.as<float>(), img, st);
+                           v->sample, v->brightness, ws.accum.as<float>(), img, st);
             hip_check(hipMemcpyAsync(rgba[k], img, 4 * (size_t)jobs[k].pixels,
                                      hipMemcpyDeviceToHost, st), "read image");
         }

@@ -14,7 +14,11 @@
 //      still has samples to take: the same expand / cast / shade launches
 //      as hpt_render, over the union of their live paths;
 //   4. requests whose last sample finished are resolved, copied to the
-//      caller's buffers and released; waiters are woken.
+//      caller's buffers and released; waiters are woken.  A progressive
+//      request (HPT_SERVE_PROGRESSIVE) is also resolved after every macro
+//      pass that advanced it, into an image the caller reads with
+//      hpt_server_peek: the progressive preview the reference's GUI gives by
+//      re-rendering its view (src/hptracing.cpp:60-83, 124-130).
 // Requests arriving while others run join at the next macro pass.  Paths
 // carry their own random streams, so a request's image does not depend on
 // what it shared the GPU with.
@@ -37,11 +41,12 @@ constexpr int kMaxSlots = 1024;
 constexpr int kServerPasses = 8;
 static_assert(kServerPasses <= kPassSlices, "passes in flight must use distinct slices");
 // Device bytes per camera-ray slot of the arena: primary org/dir/thr (48),
-// primary hit id/t (8), cached hit (8), hit list (4), RGBA (4), an
-// accumulator per pass slice (12 each), and per pass of a macro pass two
-// path buffers (2 x 56) and hit id/t of the live paths (8).
+// primary hit id/t (8), cached hit (8), hit list (4), RGBA (4), the folded
+// accumulator (12), an accumulator per pass slice (12 each), and per pass
+// of a macro pass two path buffers (2 x 56) and hit id/t of the live paths
+// (8).
 constexpr int64_t kBytesPerRay =
-    48 + 8 + 8 + 4 + 4 + kPassSlices * 12 + kServerPasses * (112 + 8);
+    48 + 8 + 8 + 4 + 4 + 12 + kPassSlices * 12 + kServerPasses * (112 + 8);
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -59,6 +64,10 @@ struct Request {
     int n1 = 0;
     int next_pass = 0;
     int status = 0;     // 0 queued, 1 active, 2 done, 3 failed
+    bool progressive = false;
+    std::vector<uint8_t> stage;    // worker side: the image just resolved
+    std::vector<uint8_t> preview;  // under the server mutex: the latest image
+    int published = 0;             // passes in `preview` (under the mutex)
     double t_submit = 0, t_admit = 0, t_done = 0;
     std::string error;
 };
@@ -108,6 +117,7 @@ struct hpt_server {
     int max_depth = HPT_MAX_DEPTH;
     hipStream_t stream = nullptr;
     Workspace ws;           // arena: indexed by slot of the camera-ray range
+    DevMem folded;          // the slices of a request summed, per camera ray
     RangeAlloc alloc;
     std::vector<int> free_slots;
 
@@ -128,7 +138,9 @@ struct hpt_server {
     void run();
     void admit_locked(std::vector<Request *> &admitted);
     void start_requests(const std::vector<Request *> &admitted);
-    void one_pass(std::vector<Request *> &finished);
+    void one_pass(std::vector<Request *> &finished, std::vector<Request *> &progressed);
+    void resolve(Request *r, int samples);
+    void publish(const std::vector<Request *> &progressed);
     void finish(const std::vector<Request *> &finished);
 };
 
@@ -197,7 +209,7 @@ void hpt_server::start_requests(const std::vector<Request *> &admitted) {
     }
 }
 
-void hpt_server::one_pass(std::vector<Request *> &finished) {
+void hpt_server::one_pass(std::vector<Request *> &finished, std::vector<Request *> &progressed) {
     hipStream_t st = stream;
     std::vector<Segment> segs;
     PassDesc pd{0, 0, 0};
@@ -215,6 +227,7 @@ void hpt_server::one_pass(std::vector<Request *> &finished) {
         }
         r->next_pass = end;
         if (r->next_pass >= r->v.sample) finished.push_back(r);
+        else if (r->progressive) progressed.push_back(r);
     }
     if (pd.total > 0) {
         hip_check(hipMemcpyAsync(ws.segs.p, segs.data(), sizeof(Segment) * segs.size(),
@@ -253,22 +266,55 @@ void hpt_server::one_pass(std::vector<Request *> &finished) {
     ++macro_passes;
 }
 
+// Resolve request r as an image of `samples` passes into the arena: the
+// background of the missed camera rays for that many samples, the slices
+// summed into the folded accumulator (the slices stay as they are, so the
+// request can go on), then the tone map.  After `samples` passes the image
+// equals hpt_render's for the same view with sample = samples.
+void hpt_server::resolve(Request *r, int samples) {
+    hipStream_t st = stream;
+    ViewParams *d_views = ws.views.as<ViewParams>();
+    const hpt_view &v = r->v;
+    if (r->progressive)
+        launch_misses(d_views, r->slot, r->R, ws.prim_id.as<int>(), samples, ws.accum.as<float>(),
+                      st);
+    launch_fold(d_views, r->slot, r->R, ws.accum.as<float>(), folded.as<float>(), st);
+    uchar4 *img = ws.rgba.as<uchar4>() + r->base;  // pixels <= camera rays
+    launch_resolve(d_views, r->slot, v.sample_x, v.sample_y, v.supersample_x, v.supersample_y,
+                   samples, v.brightness, folded.as<float>(), img, st);
+}
+
+void hpt_server::publish(const std::vector<Request *> &progressed) {
+    if (progressed.empty()) return;
+    for (Request *r : progressed) {
+        resolve(r, r->next_pass);
+        hip_check(hipMemcpyAsync(r->stage.data(), ws.rgba.as<uchar4>() + r->base,
+                                 4 * (size_t)r->pixels, hipMemcpyDeviceToHost, stream),
+                  "read preview");
+    }
+    hip_check(hipStreamSynchronize(stream), "preview");
+    std::lock_guard<std::mutex> lock(mu);
+    for (Request *r : progressed) {
+        r->preview = r->stage;
+        r->published = r->next_pass;
+    }
+    done_cv.notify_all();
+}
+
 void hpt_server::finish(const std::vector<Request *> &finished) {
     if (finished.empty()) return;
     hipStream_t st = stream;
-    ViewParams *d_views = ws.views.as<ViewParams>();
     for (Request *r : finished) {
-        const hpt_view &v = r->v;
-        launch_fold(d_views, r->slot, r->R, ws.accum.as<float>(), st);
-        if (r->rgba) {
-            uchar4 *img = ws.rgba.as<uchar4>() + r->base;  // pixels <= camera rays
-            launch_resolve(d_views, r->slot, v.sample_x, v.sample_y, v.supersample_x,
-                           v.supersample_y, v.brightness, ws.accum.as<float>(), img, st);
+        resolve(r, r->v.sample);
+        const uchar4 *img = ws.rgba.as<uchar4>() + r->base;
+        if (r->rgba)
             hip_check(hipMemcpyAsync(r->rgba, img, 4 * (size_t)r->pixels, hipMemcpyDeviceToHost, st),
                       "read image");
-        }
+        if (r->progressive)
+            hip_check(hipMemcpyAsync(r->stage.data(), img, 4 * (size_t)r->pixels,
+                                     hipMemcpyDeviceToHost, st), "read preview");
         if (r->accum)
-            hip_check(hipMemcpyAsync(r->accum, ws.accum.as<float>() + 3 * r->base, 12 * (size_t)r->R,
+            hip_check(hipMemcpyAsync(r->accum, folded.as<float>() + 3 * r->base, 12 * (size_t)r->R,
                                      hipMemcpyDeviceToHost, st), "read accum");
     }
     hip_check(hipStreamSynchronize(st), "resolve");
@@ -277,7 +323,7 @@ void hpt_server::finish(const std::vector<Request *> &finished) {
 void hpt_server::run() {
     hip_check(hipSetDevice(scene->device), "hipSetDevice");
     for (;;) {
-        std::vector<Request *> admitted, finished;
+        std::vector<Request *> admitted, finished, progressed;
         {
             std::unique_lock<std::mutex> lock(mu);
             wake.wait(lock, [&] { return stopping || !queue.empty() || !active.empty(); });
@@ -292,7 +338,8 @@ void hpt_server::run() {
         const double t0 = now_s();
         try {
             if (!admitted.empty()) start_requests(admitted);
-            one_pass(finished);
+            one_pass(finished, progressed);
+            publish(progressed);
             finish(finished);
         } catch (const Fail &e) {
             std::lock_guard<std::mutex> lock(mu);
@@ -312,6 +359,10 @@ void hpt_server::run() {
         for (Request *r : finished) {
             r->status = 2;
             r->t_done = t;
+            if (r->progressive) {
+                r->preview = r->stage;
+                r->published = r->v.sample;
+            }
             alloc.give(r->base, r->R);
             free_slots.push_back(r->slot);
             active.erase(std::find(active.begin(), active.end(), r));
@@ -337,6 +388,7 @@ int hpt_server_create(hpt_scene *scene, const hpt_server_opts *opts, hpt_server 
         sv->ws.ensure(sv->capacity, sv->capacity, 1, HPT_MAX_DEPTH, kMaxSlots,
                       sv->capacity * kServerPasses);
         sv->ws.segs.reserve(sizeof(Segment) * kMaxSlots * kServerPasses);
+        sv->folded.reserve(12 * (size_t)sv->capacity);
         sv->alloc.init(sv->capacity);
         for (int k = kMaxSlots - 1; k >= 0; --k) sv->free_slots.push_back(k);
         hip_check(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -356,8 +408,14 @@ int hpt_server_create(hpt_scene *scene, const hpt_server_opts *opts, hpt_server 
 
 int hpt_server_submit(hpt_server *sv, const hpt_view *v, uint8_t *rgba, float *accum,
                       int64_t *id) {
+    return hpt_server_submit_ex(sv, v, rgba, accum, 0u, id);
+}
+
+int hpt_server_submit_ex(hpt_server *sv, const hpt_view *v, uint8_t *rgba, float *accum,
+                         unsigned flags, int64_t *id) {
     return guarded([&] {
         if (!sv || !v || !id) fail(HPT_ERR_INVALID, "null argument");
+        if (flags & ~HPT_SERVE_PROGRESSIVE) fail(HPT_ERR_INVALID, "unknown flags");
         if (v->sample_x <= 0 || v->sample_y <= 0 || v->supersample_x <= 0 || v->supersample_y <= 0)
             fail(HPT_ERR_INVALID, "image and supersample sizes must be positive");
         if (v->sample <= 0 || v->depth <= 0 || v->depth > HPT_MAX_DEPTH)
@@ -372,6 +430,10 @@ int hpt_server_submit(hpt_server *sv, const hpt_view *v, uint8_t *rgba, float *a
         r->sy = (int64_t)v->sample_y * v->supersample_y;
         r->R = r->sx * r->sy;
         r->pixels = (int64_t)v->sample_x * v->sample_y;
+        if (flags & HPT_SERVE_PROGRESSIVE) {
+            r->progressive = true;
+            r->stage.assign(4 * (size_t)r->pixels, 0);
+        }
         if (r->R > sv->capacity)
             fail(HPT_ERR_BUDGET, "request needs " + std::to_string(r->R * kBytesPerRay) +
                                      " device bytes, more than the server's budget");
@@ -420,6 +482,36 @@ int hpt_server_wait(hpt_server *sv, int64_t id, double timeout_s, hpt_request_in
     });
     if (rc == HPT_OK && code != HPT_OK) {
         g_error = "request not finished before the timeout";
+        return code;
+    }
+    return rc;
+}
+
+int hpt_server_peek(hpt_server *sv, int64_t id, int32_t min_passes, double timeout_s,
+                    uint8_t *rgba, int32_t *passes) {
+    int code = HPT_OK;
+    int rc = guarded([&] {
+        if (!sv) fail(HPT_ERR_INVALID, "null server");
+        std::unique_lock<std::mutex> lock(sv->mu);
+        auto it = sv->requests.find(id);
+        if (it == sv->requests.end()) fail(HPT_ERR_INVALID, "unknown request id");
+        Request *r = it->second.get();
+        if (!r->progressive) fail(HPT_ERR_INVALID, "not a progressive request");
+        auto ready = [&] {
+            return r->published >= min_passes || r->status >= 2 || !sv->fatal.empty();
+        };
+        if (timeout_s < 0) {
+            sv->done_cv.wait(lock, ready);
+        } else if (!sv->done_cv.wait_for(lock, std::chrono::duration<double>(timeout_s), ready)) {
+            code = HPT_ERR_TIMEOUT;
+        }
+        if (r->status == 3 || (r->status < 2 && !sv->fatal.empty()))
+            fail(HPT_ERR_HIP, "request failed: " + (r->error.empty() ? sv->fatal : r->error));
+        if (passes) *passes = r->published;
+        if (rgba && r->published > 0) std::memcpy(rgba, r->preview.data(), r->preview.size());
+    });
+    if (rc == HPT_OK && code != HPT_OK) {
+        g_error = "fewer passes published before the timeout";
         return code;
     }
     return rc;

@@ -158,7 +158,9 @@ class RenderServer:
 
     ``submit(view)`` queues a request and returns its id; ``wait(id)``
     blocks until the image is in the returned buffer.  Requests submitted
-    while others render join them at the next pass."""
+    while others render join them at the next pass.  A request submitted
+    with ``progressive=True`` also publishes its image after every macro
+    pass; ``peek(id)`` returns the latest one and its pass count."""
 
     def __init__(self, runner: TraceRunner, device_budget_bytes: int = 4 << 30):
         self.runner = runner
@@ -168,16 +170,30 @@ class RenderServer:
                                                     C.byref(self._h)))
         self._buffers = {}
 
-    def submit(self, v: N.View, return_accum=False) -> int:
+    def submit(self, v: N.View, return_accum=False, progressive=False) -> int:
         img = np.zeros((v.sample_y, v.sample_x, 4), np.uint8)
         acc = np.zeros((v.sample_x * v.supersample_x * v.sample_y * v.supersample_y, 3),
                        np.float32) if return_accum else None
         rid = C.c_int64()
-        N.check_device(N.device().hpt_server_submit(
+        flags = N.HPT_SERVE_PROGRESSIVE if progressive else 0
+        N.check_device(N.device().hpt_server_submit_ex(
             self._h, C.byref(v), img.ctypes.data, acc.ctypes.data if acc is not None else None,
-            C.byref(rid)))
+            flags, C.byref(rid)))
         self._buffers[rid.value] = (img, acc)
         return rid.value
+
+    def peek(self, rid: int, min_passes: int = 0, timeout_s: float = -1.0):
+        """Latest published image of a progressive request once it has at
+        least `min_passes` passes (or has finished): (rgba or None, passes).
+        With a timeout that expires first, the latest image so far."""
+        img, _ = self._buffers[rid]
+        out = np.zeros_like(img)
+        passes = C.c_int32(0)
+        rc = N.device().hpt_server_peek(self._h, rid, min_passes, timeout_s, out.ctypes.data,
+                                        C.byref(passes))
+        if rc not in (N.HPT_OK, N.HPT_ERR_TIMEOUT):
+            N.check_device(rc)
+        return (out if passes.value > 0 else None), passes.value
 
     def wait(self, rid: int, timeout_s: float = -1.0):
         """Returns (rgba, accum or None, info dict)."""

@@ -6,7 +6,7 @@
  * kernel argument lists of src/hp/cl_src/kernel.cl.  Everything here is plain
  * C: opaque handles, pointers and sizes, `int` status codes and a
  * thread-local error string.  No HIP or torch type appears in a signature;
- * a HIP stream is passed as `void *` (NULL = the library's own stream).
+ * a HIP stream is passed as `void *` (NULL = HIP's null stream).
  *
  * Two shared libraries implement it:
  *   libhpt_host.so  - CPU side: OBJ/MTL loading, light table, textures,
@@ -332,6 +332,26 @@ int hpt_server_create(hpt_scene *scene, const hpt_server_opts *opts,
  * it.  HPT_ERR_BUDGET if the request can never fit the arena. */
 int hpt_server_submit(hpt_server *server, const hpt_view *view, uint8_t *rgba,
                       float *accum, int64_t *request_id);
+
+/* Progressive preview: the reference's GUI shows a view by re-rendering it
+ * whenever the camera moves (src/hptracing.cpp:60-83, 124-130); here a
+ * progressive request publishes its image after every macro pass, and the
+ * caller reads the latest one while the request goes on. */
+#define HPT_SERVE_PROGRESSIVE 1u
+/* hpt_server_submit with HPT_SERVE_* flags. */
+int hpt_server_submit_ex(hpt_server *server, const hpt_view *view, uint8_t *rgba,
+                         float *accum, unsigned flags, int64_t *request_id);
+/* For a progressive request: wait until it has published an image of at
+ * least min_passes sample passes, or finished, or timeout_s seconds passed
+ * (< 0: no limit); then copy its latest image (W*H*4 bytes) into rgba (may
+ * be NULL) and that image's pass count into *passes (0: none yet).  The
+ * image of S passes equals hpt_render's for the same view with sample = S,
+ * bit for bit.  HPT_ERR_TIMEOUT if fewer passes were published in time (the
+ * outputs then hold the latest image).  Valid until hpt_server_wait retires
+ * the request. */
+int hpt_server_peek(hpt_server *server, int64_t request_id, int32_t min_passes,
+                    double timeout_s, uint8_t *rgba, int32_t *passes);
+
 /* Wait up to timeout_s seconds (< 0: forever).  HPT_ERR_TIMEOUT if still
  * running; otherwise the request is retired and its id forgotten. */
 int hpt_server_wait(hpt_server *server, int64_t request_id, double timeout_s,

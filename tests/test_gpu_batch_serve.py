@@ -87,3 +87,67 @@ def test_server_budget_and_timeout(runner):
     if rc == N.HPT_ERR_TIMEOUT:
         srv.wait(rid)
     srv.close()
+
+
+@pytest.mark.parametrize("vr", [False, True])
+def test_progressive_request_equals_render_at_each_pass_count(runner, vr):
+    """A progressive request's image after S passes is hpt_render's image of
+    the same view with sample = S, bit for bit -- including the background
+    of camera rays that miss (the equirectangular view looks out of the
+    room's open side) -- and its last image is the request's result."""
+    from hptracing_amd.runner import RenderServer
+    kw = dict(sample_x=40, sample_y=24, sample=20, seed=21)
+    if vr:
+        kw.update(height=-1.0, right_dir=(3.0, 0.0, 0.0), view_p=(5.0, 5.0, 5.0))
+    v = _view(**kw)
+    srv = RenderServer(runner, device_budget_bytes=64 << 20)
+    rid = srv.submit(v, progressive=True)
+    seen, passes = {}, 0
+    while passes < v.sample:
+        img, passes = srv.peek(rid, min_passes=passes + 1)
+        seen[passes] = img
+    final, _, _ = srv.wait(rid)
+    srv.close()
+    assert set(seen) <= {8, 16, 20} and 20 in seen
+    assert np.array_equal(seen[20], final)
+    for s, img in seen.items():
+        ref = runner.render(_view(**{**kw, "sample": s}))
+        assert np.array_equal(img, ref), f"image after {s} passes"
+    if vr:
+        bg = np.round(255 * (1 - np.exp(-np.array([0.05, 0.1, 0.2])))).astype(np.uint8)
+        assert (final[..., :3] == bg).all(axis=-1).any(), "no background pixel"
+
+
+def test_peek_rules(runner):
+    from hptracing_amd.runner import RenderServer
+    srv = RenderServer(runner, device_budget_bytes=64 << 20)
+    plain = srv.submit(_view(sample=2))
+    with pytest.raises(N.HptError):
+        srv.peek(plain)
+    rid = srv.submit(_view(sample=12, seed=5), progressive=True)
+    img, passes = srv.peek(rid, min_passes=12)
+    assert passes == 12 and img is not None
+    srv.wait(plain)
+    srv.wait(rid)
+    srv.close()
+
+
+def test_preview_session_frames(runner):
+    """The GUI counterpart: frames refine until the full sample count, the
+    last equals hpt_render of the session's view; a key press changes the
+    view and the next frames show it."""
+    from hptracing_amd.preview import PreviewSession
+    from hptracing_amd.runner import RenderServer
+    srv = RenderServer(runner, device_budget_bytes=64 << 20)
+    r = scenes.ROOM_VIEW
+    s = PreviewSession(srv, view=r["view_p"], up=r["top_dir"], right=r["right_dir"],
+                       width=32, height=24, sample=17, depth=4, seed=3)
+    frames = list(s.frames())
+    assert [p for _, p in frames][-1] == 17
+    assert all(a < b for a, b in zip([p for _, p in frames], [p for _, p in frames][1:]))
+    assert np.array_equal(frames[-1][0], runner.render(s.hpt_view()))
+    assert s.key("d")
+    moved = s.render()
+    assert np.array_equal(moved, runner.render(s.hpt_view()))
+    assert not np.array_equal(moved, frames[-1][0])
+    srv.close()
