@@ -83,7 +83,7 @@ def test_autotune_keeps_results(room):
 
 
 def test_pass_at_a_time_same_bits(room):
-    """Eight passes per launch sequence or one at a time (the reference's
+    """Sixteen passes per launch sequence or one at a time (the reference's
     launch structure): the accumulator slices make the sums identical."""
     from hptracing_amd.runner import TraceRunner, make_view
     r = TraceRunner(room)
