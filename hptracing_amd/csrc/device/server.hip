@@ -68,6 +68,7 @@ struct Request {
     std::vector<uint8_t> stage;    // worker side: the image just resolved
     std::vector<uint8_t> preview;  // under the server mutex: the latest image
     int published = 0;             // passes in `preview` (under the mutex)
+    int peekers = 0;               // hpt_server_peek calls inside (under the mutex)
     double t_submit = 0, t_admit = 0, t_done = 0;
     std::string error;
 };
@@ -478,6 +479,7 @@ int hpt_server_wait(hpt_server *sv, int64_t id, double timeout_s, hpt_request_in
         fill_info(*r, info);
         if (r->status == 3 || (r->status < 2 && !sv->fatal.empty()))
             fail(HPT_ERR_HIP, "request failed: " + (r->error.empty() ? sv->fatal : r->error));
+        sv->done_cv.wait(lock, [&] { return r->peekers == 0; });
         sv->requests.erase(it);  // results delivered; forget the request
     });
     if (rc == HPT_OK && code != HPT_OK) {
@@ -500,11 +502,14 @@ int hpt_server_peek(hpt_server *sv, int64_t id, int32_t min_passes, double timeo
         auto ready = [&] {
             return r->published >= min_passes || r->status >= 2 || !sv->fatal.empty();
         };
+        // hpt_server_wait does not retire the request while a peek is inside.
+        ++r->peekers;
         if (timeout_s < 0) {
             sv->done_cv.wait(lock, ready);
         } else if (!sv->done_cv.wait_for(lock, std::chrono::duration<double>(timeout_s), ready)) {
             code = HPT_ERR_TIMEOUT;
         }
+        if (--r->peekers == 0) sv->done_cv.notify_all();
         if (r->status == 3 || (r->status < 2 && !sv->fatal.empty()))
             fail(HPT_ERR_HIP, "request failed: " + (r->error.empty() ? sv->fatal : r->error));
         if (passes) *passes = r->published;

@@ -357,7 +357,8 @@ int hpt_server_peek(hpt_server *server, int64_t request_id, int32_t min_passes,
 int hpt_server_wait(hpt_server *server, int64_t request_id, double timeout_s,
                     hpt_request_info *info);
 int hpt_server_stats_get(hpt_server *server, hpt_server_stats *out);
-/* Finish every queued and running request, then stop the worker. */
+/* Finish every queued and running request, then stop the worker.  No other
+ * call on the server may still be running (wait and peek included). */
 int hpt_server_destroy(hpt_server *server);
 
 #ifdef __cplusplus
