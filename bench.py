@@ -23,7 +23,8 @@ Besides the timed render the script reports:
   cpu_baseline  the float32 C restatement of the reference kernels (oracle/)
                 on the host cores, on a bounded sample of the same camera
                 rays at the same spp.
-  cast_only     the ray-cast stage in isolation (hpt_cast): 2^22 rays with
+  cast_only     the ray-cast stage in isolation (hpt_cast), with the default
+                launch forms and after hpt_autotune_cast: 2^22 rays with
                 origins uniform in the scene box and directions uniform on
                 the sphere through a seeded triangle soup (centres uniform
                 in the unit cube, edges of log-uniform length in
@@ -347,6 +348,19 @@ def bench_cast_only(n_tris):
         torch.cuda.synchronize()
         times.append(e0.elapsed_time(e1))
     ms = float(np.median(times))
+    # The same batch after hpt_autotune_cast (the defaults are tuned for the
+    # render's rays).
+    chosen = runner.autotune_cast(org_t, dir_t)
+    tuned = []
+    for _ in range(5):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        runner.cast(org_t, dir_t)
+        e1.record()
+        torch.cuda.synchronize()
+        tuned.append(e0.elapsed_time(e1))
+    ms_tuned = float(np.median(tuned))
     _, _, cnt = runner.cast(org_t, dir_t, counted=True)
     torch.cuda.synchronize()
     c = cnt.sum(dim=0, dtype=torch.int64).cpu().numpy()
@@ -354,7 +368,7 @@ def bench_cast_only(n_tris):
     gbs = algo / (ms * 1e-3) / 1e9
     log(f"[bench] cast only: {kd.n_triangles} tris ({len(kd.leaf_data)} leaf ints), {n} rays, "
         f"{ms:.2f} ms, {n / ms / 1e3:.1f} Mrays/s, {algo / n:.0f} algorithmic B/ray, "
-        f"{gbs:.0f} GB/s; setup {time.time() - t0:.1f}s")
+        f"{gbs:.0f} GB/s; autotuned {n / ms_tuned / 1e3:.1f} Mrays/s; setup {time.time() - t0:.1f}s")
     return {"workload": f"hpt_cast, {n} rays through a {kd.n_triangles}-triangle soup",
             "triangles": kd.n_triangles, "rays": n, "ms": round(ms, 3),
             "mrays_s": round(n / ms / 1e3, 2),
@@ -362,7 +376,10 @@ def bench_cast_only(n_tris):
             "leaves_per_ray": round(int(c[1]) / n, 1),
             "tri_tests_per_ray": round(int(c[2]) / n, 1),
             "algorithmic_bytes": algo, "achieved_gbs": round(gbs, 1),
-            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "autotuned_ms": round(ms_tuned, 3),
+            "autotuned_mrays_s": round(n / ms_tuned / 1e3, 2),
+            "autotuned_tuning": chosen}
 
 
 def cpu_baseline(kd, view, spp, target_s, R):

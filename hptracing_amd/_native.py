@@ -213,6 +213,7 @@ def device() -> C.CDLL:
                                          C.POINTER(vp), C.POINTER(RenderStats), C.c_uint]
         lib.hpt_scene_set_arithmetic.argtypes = [vp, C.c_int]
         lib.hpt_autotune.argtypes = [vp, C.POINTER(View), C.c_int, C.POINTER(Tuning)]
+        lib.hpt_autotune_cast.argtypes = [vp, vp, vp, i64, i64, C.POINTER(Tuning)]
         lib.hpt_scene_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         lib.hpt_scene_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         lib.hpt_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]

@@ -286,6 +286,89 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
     }
 }
 
+// Coordinate descent over the scene's launch forms from its current
+// setting: each knob in turn takes the value with the shortest cast time
+// (measure(true), in ms) while the others are held, two sweeps; then, if
+// `shade`, the shade grid by measure(false).  Leaves the best setting on
+// the scene and returns it.  No setting changes a result.
+template <class Measure>
+hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
+    hpt_tuning base;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        base.fast_division = s->dev.fast_div;
+        base.traversal = s->dev.traversal;
+        base.refill_threshold = s->dev.refill;
+        base.node_steps = s->dev.node_steps;
+        base.fetch = s->dev.fetch;
+        base.walk_min = s->dev.walk_min;
+        base.cast_blocks_per_cu = s->cast_grid / s->n_cus;
+        base.shade_blocks_per_cu = s->shade_grid / s->n_cus;
+    }
+    auto apply = [&](const hpt_tuning &t) {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->dev.fast_div = t.fast_division;
+        s->dev.traversal = t.traversal;
+        s->dev.refill = t.refill_threshold;
+        s->dev.node_steps = t.node_steps;
+        s->dev.fetch = t.fetch;
+        s->dev.walk_min = t.walk_min;
+        s->cast_grid = std::max(1, t.cast_blocks_per_cu) * s->n_cus;
+        s->shade_grid = std::max(1, t.shade_blocks_per_cu) * s->n_cus;
+    };
+    auto timed = [&](const hpt_tuning &t, bool cast) {
+        apply(t);
+        return measure(cast);
+    };
+    timed(base, true);  // warm caches and code
+    hpt_tuning best = base;
+    double best_ms = timed(best, true);
+    auto try_knob = [&](auto set, std::initializer_list<int> values) {
+        const hpt_tuning start = best;
+        for (int v : values) {
+            hpt_tuning t = start;
+            set(t, v);
+            if (!valid_fetch(t.fetch)) continue;
+            const double ms = timed(t, true);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    };
+    // The lane-refill knobs only matter for traversal 2.
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        try_knob([](hpt_tuning &t, int v) { t.traversal = v; }, {0, 1, 2});
+        if (best.traversal == 2) {
+            try_knob([](hpt_tuning &t, int v) { t.fetch = v; },
+                     {0, 1, 4, 5, 12, 13, 20, 21, 28, 29, 52, 60, 84, 92, 116, 124});
+            try_knob([](hpt_tuning &t, int v) { t.refill_threshold = v; },
+                     {8, 16, 24, 32, 48, 64});
+            try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {8, 16, 24, 32, 48, 64});
+            try_knob([](hpt_tuning &t, int v) { t.walk_min = v; }, {0, 16, 24, 32, 40});
+        }
+        try_knob([](hpt_tuning &t, int v) { t.fast_division = v; }, {0, 1});
+        try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {3, 4, 5, 6, 7, 8});
+    }
+    if (shade) {
+        const int socc = std::max(1, base.shade_blocks_per_cu);
+        double best_shade = 1e30;
+        int best_sbpc = socc;
+        for (int sb : {std::max(1, socc / 2), socc, std::min(64, socc * 2)}) {
+            hpt_tuning t = best;
+            t.shade_blocks_per_cu = sb;
+            const double ms = timed(t, false);
+            if (ms < best_shade) {
+                best_shade = ms;
+                best_sbpc = sb;
+            }
+        }
+        best.shade_blocks_per_cu = best_sbpc;
+    }
+    apply(best);
+    return best;
+}
+
 }  // namespace
 
 extern "C" {
@@ -322,31 +405,7 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             v.sample_x /= 2;
             v.sample_y /= 2;
         }
-        hpt_tuning base;
-        {
-            std::lock_guard<std::mutex> lock(s->mu);
-            base.fast_division = s->dev.fast_div;
-            base.traversal = s->dev.traversal;
-            base.refill_threshold = s->dev.refill;
-            base.node_steps = s->dev.node_steps;
-            base.fetch = s->dev.fetch;
-            base.walk_min = s->dev.walk_min;
-            base.cast_blocks_per_cu = s->cast_grid / s->n_cus;
-            base.shade_blocks_per_cu = s->shade_grid / s->n_cus;
-        }
-        auto apply = [&](const hpt_tuning &t) {
-            std::lock_guard<std::mutex> lock(s->mu);
-            s->dev.fast_div = t.fast_division;
-            s->dev.traversal = t.traversal;
-            s->dev.refill = t.refill_threshold;
-            s->dev.node_steps = t.node_steps;
-            s->dev.fetch = t.fetch;
-            s->dev.walk_min = t.walk_min;
-            s->cast_grid = std::max(1, t.cast_blocks_per_cu) * s->n_cus;
-            s->shade_grid = std::max(1, t.shade_blocks_per_cu) * s->n_cus;
-        };
-        auto measure = [&](const hpt_tuning &t, bool cast) {
-            apply(t);
+        auto measure = [&](bool cast) {
             hpt_render_stats st;
             double best = 1e30;
             for (int rep = 0; rep < 2; ++rep) {
@@ -355,53 +414,36 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
             }
             return best;
         };
-        measure(base, true);  // warm caches and code
-        // Coordinate descent from the current setting: each knob in turn
-        // takes the value that gives the shortest cast time with the others
-        // held, two sweeps.  The lane-refill knobs only matter for
-        // traversal 2.
-        hpt_tuning best = base;
-        double best_ms = measure(best, true);
-        auto try_knob = [&](auto set, std::initializer_list<int> values) {
-            const hpt_tuning start = best;
-            for (int v : values) {
-                hpt_tuning t = start;
-                set(t, v);
-                if (!valid_fetch(t.fetch)) continue;
-                const double ms = measure(t, true);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = t;
-                }
+        const hpt_tuning best = tune(s, measure, true);
+        if (chosen) *chosen = best;
+    });
+}
+
+int hpt_autotune_cast(hpt_scene *s, const float *org, const float *dir, int64_t n,
+                      int64_t max_rays, hpt_tuning *chosen) {
+    return guarded([&] {
+        if (!s || !org || !dir) fail(HPT_ERR_INVALID, "null argument");
+        if (n <= 0 || n > ((int64_t)1 << 30)) fail(HPT_ERR_INVALID, "ray count out of range");
+        if (max_rays <= 0) max_rays = (int64_t)1 << 22;
+        const int64_t m = std::min(n, max_rays);
+        DevMem ids, ts;
+        ids.reserve(4 * (size_t)m);
+        ts.reserve(4 * (size_t)m);
+        hipStream_t st = s->stream;
+        auto measure = [&](bool) {
+            double best = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                KernelTimer timer(true, st);
+                timer.begin();
+                launch_cast(s->dev, org, dir, nullptr, nullptr, CountRef{}, m, s->counter(),
+                            ids.as<int>(), ts.as<float>(), nullptr, nullptr, s->cast_grid, st);
+                timer.end();
+                hip_check(hipStreamSynchronize(st), "autotune cast");
+                best = std::min(best, timer.total_ms());
             }
+            return best;
         };
-        for (int sweep = 0; sweep < 2; ++sweep) {
-            try_knob([](hpt_tuning &t, int v) { t.traversal = v; }, {0, 1, 2});
-            if (best.traversal == 2) {
-                try_knob([](hpt_tuning &t, int v) { t.fetch = v; },
-                         {0, 1, 4, 5, 12, 13, 20, 21, 28, 29, 52, 60, 84, 92, 116, 124});
-                try_knob([](hpt_tuning &t, int v) { t.refill_threshold = v; },
-                         {8, 16, 24, 32, 48, 64});
-                try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {8, 16, 24, 32, 48, 64});
-                try_knob([](hpt_tuning &t, int v) { t.walk_min = v; }, {0, 16, 24, 32, 40});
-            }
-            try_knob([](hpt_tuning &t, int v) { t.fast_division = v; }, {0, 1});
-            try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {3, 4, 5, 6, 7, 8});
-        }
-        const int socc = std::max(1, base.shade_blocks_per_cu);
-        double best_shade = 1e30;
-        int best_sbpc = socc;
-        for (int sb : {std::max(1, socc / 2), socc, std::min(64, socc * 2)}) {
-            hpt_tuning t = best;
-            t.shade_blocks_per_cu = sb;
-            const double ms = measure(t, false);
-            if (ms < best_shade) {
-                best_shade = ms;
-                best_sbpc = sb;
-            }
-        }
-        best.shade_blocks_per_cu = best_sbpc;
-        apply(best);
+        const hpt_tuning best = tune(s, measure, false);
         if (chosen) *chosen = best;
     });
 }

@@ -72,6 +72,18 @@ class TraceRunner:
         N.check_device(N.device().hpt_autotune(self._h, C.byref(v), max_rays, C.byref(t)))
         return {k: getattr(t, k) for k, _ in N.Tuning._fields_}
 
+    def autotune_cast(self, org, dir, max_rays: int = 0) -> dict:
+        """Pick the fastest tuning for hpt_cast on a ray batch ((n,3) float32,
+        numpy or torch); uses at most max_rays of the rays (0: 2^22)."""
+        import torch
+        org_t = torch.as_tensor(org, dtype=torch.float32).cuda().contiguous()
+        dir_t = torch.as_tensor(dir, dtype=torch.float32).cuda().contiguous()
+        torch.cuda.synchronize()
+        t = N.Tuning()
+        N.check_device(N.device().hpt_autotune_cast(self._h, org_t.data_ptr(), dir_t.data_ptr(),
+                                                    org_t.shape[0], max_rays, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in N.Tuning._fields_}
+
     def set_arithmetic(self, rocm_opencl: bool) -> None:
         """dot/cross of the intersection test: literal (default) or the fused
         forms of ROCm's OpenCL builtins."""

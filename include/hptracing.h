@@ -282,6 +282,14 @@ int hpt_render_ex(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
 int hpt_autotune(hpt_scene *scene, const hpt_view *view, int max_rays,
                  hpt_tuning *chosen);
 
+/* The same tuning for ray casts: the knobs of hpt_autotune (the shade grid
+ * aside) timed on hpt_cast of the first min(n, max_rays) rays of a batch in
+ * device memory (xyz triples; max_rays <= 0: 2^22).  Keeps the result on
+ * the scene and returns it in `chosen` (may be NULL).  No reference
+ * counterpart. */
+int hpt_autotune_cast(hpt_scene *scene, const float *org, const float *dir,
+                      int64_t n, int64_t max_rays, hpt_tuning *chosen);
+
 /* Render several views of one scene together (no reference counterpart; the
  * reference renders one view per TraceRunner::run call).  The camera rays
  * of all views are cast in one launch, and every pass launches the

@@ -152,3 +152,18 @@ def test_cast_exact_ties(oracle_mod, fast):
     org, d = scenes.random_rays(30000, (0, 0, 0), (1, 1, 1), seed=14)
     ids, _ = _check(kd, org, d, oracle_mod, fast=fast)
     assert (ids >= 0).mean() > 0.05
+
+
+def test_autotune_cast_keeps_results(soup_small):
+    """hpt_autotune_cast picks launch forms on a ray batch; none changes a bit."""
+    import torch
+    r = _runner(soup_small, (1, 2, 48, 24, 29))
+    org, d = scenes.random_rays(30000, (0, 0, 0), (1, 1, 1), seed=15)
+    ids0, t0 = r.cast(org, d)
+    torch.cuda.synchronize()
+    chosen = r.autotune_cast(org, d)
+    assert chosen == r.get_tuning()
+    ids1, t1 = r.cast(org, d)
+    torch.cuda.synchronize()
+    assert torch.equal(ids0, ids1)
+    assert torch.equal(t0.view(torch.int32), t1.view(torch.int32))
