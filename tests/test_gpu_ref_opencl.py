@@ -109,19 +109,37 @@ def test_cast_literal_arithmetic_same_hits(which, request):
     assert within >= (0.99 if which == "room" else 0.90)
 
 
+@pytest.fixture(scope="module")
+def full_scene():
+    """The 1080p render's scene: room, 500k-triangle water grid, textured
+    surface of revolution, mirror sphere (721 316 triangles)."""
+    import tempfile
+    from hptracing_amd.scene import KDTree
+    path = scenes.cached(os.path.join(tempfile.gettempdir(), "hpt_scenes"), scenes.write_full, 0,
+                         water_side=500)
+    return KDTree(path, os.path.dirname(path) + "/")
+
+
 @needs_ref
-def test_full_size_batch_against_reference_kernel(soup_200k):
-    """The ray-cast workload at full batch size -- 2^22 rays, origins uniform
-    in the scene's box, directions uniform on the sphere -- through the
-    200 000-triangle soup: every hit triangle and every distance equals the
+@pytest.mark.parametrize("which", ["soup200k", "full1080p"])
+def test_full_size_batch_against_reference_kernel(which, request):
+    """2^22 rays through the ray-cast workload's 200 000-triangle soup
+    (origins uniform in its box) and through the 1080p render's scene
+    (origins uniform in the room, as bounce rays start), directions uniform
+    on the sphere: every hit triangle and every distance equals the
     reference kernel's (ROCm OpenCL arithmetic).  Both kernels' device times
     on the same batch are printed (the reference kernel as compiled for the
     parity comparison: -O3, no contraction, correctly rounded division)."""
     import torch
     from hptracing_amd.runner import TraceRunner
-    kd = soup_200k
-    pts = kd.points[:, :3]
-    org, d = scenes.random_rays(1 << 22, pts.min(0), pts.max(0), seed=41)
+    if which == "soup200k":
+        kd = request.getfixturevalue("soup_200k")
+        pts = kd.points[:, :3]
+        lo, hi = pts.min(0), pts.max(0)
+    else:
+        kd = request.getfixturevalue("full_scene")
+        lo, hi = np.full(3, 0.05), np.full(3, scenes.ROOM - 0.05)
+    org, d = scenes.random_rays(1 << 22, lo, hi, seed=41)
     r = TraceRunner(kd)
     r.set_arithmetic(True)
     org_t = torch.as_tensor(org).cuda()
@@ -140,8 +158,9 @@ def test_full_size_batch_against_reference_kernel(soup_200k):
     ref_geo, ref_t, ref_ms = ref_cast(kd, org, d, timed=True)
     ours_geo = np.where(ids[:, None] >= 0, kd.geometries[np.maximum(ids, 0)], -1)
     n = len(ids)
-    print(f"\n[ref-opencl] 2^22 rays, 200k-triangle soup: reference kdtree_intersect "
-          f"{ref_ms:.2f} ms ({n / ref_ms / 1e3:.1f} Mrays/s), hpt_cast {best:.2f} ms "
-          f"({n / best / 1e3:.1f} Mrays/s), {ref_ms / best:.1f}x; hits {(ids >= 0).sum()}")
+    print(f"\n[ref-opencl] 2^22 rays, {which} ({kd.n_triangles} tris): reference "
+          f"kdtree_intersect {ref_ms:.2f} ms ({n / ref_ms / 1e3:.1f} Mrays/s), hpt_cast "
+          f"{best:.2f} ms ({n / best / 1e3:.1f} Mrays/s), {ref_ms / best:.1f}x; "
+          f"hits {(ids >= 0).sum()}")
     assert (ours_geo == ref_geo).all()
     assert np.array_equal(t.view(np.int32), ref_t.view(np.int32))
