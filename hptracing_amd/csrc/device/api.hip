@@ -116,6 +116,7 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         }
         if (entries > INT32_MAX / 3) fail(HPT_ERR_INVALID, "too many leaf entries");
         recs.resize(12 * std::max<int64_t>(entries, 1));
+        std::vector<int32_t> leaf_ids(std::max<int64_t>(entries, 1), 0);
         int64_t rec = 0;
         for (int64_t i = 0; i < nn; ++i) {
             const hpt_kd_node &nd = nodes[i];
@@ -139,6 +140,7 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
                     std::memcpy(&r[3], &tri, 4);
                     r[4] = pb[0] - pa[0]; r[5] = pb[1] - pa[1]; r[6] = pb[2] - pa[2]; r[7] = 0.f;
                     r[8] = pc[0] - pa[0]; r[9] = pc[1] - pa[1]; r[10] = pc[2] - pa[2]; r[11] = 0.f;
+                    leaf_ids[rec] = tri;
                     ++rec;
                 }
             } else {
@@ -159,6 +161,25 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         int64_t total = 0;
         upload(s->nodes, pnodes.data(), pnodes.size(), total);
         upload(s->tris, recs.data(), recs.size(), total);
+        // The same records once per triangle, with the leaf lists as
+        // triangle ids (the indirect fetch form): 48 bytes per triangle plus
+        // 4 per leaf entry instead of 48 per leaf entry.
+        {
+            std::vector<float> trecs(12 * std::max<int64_t>(nt, 1), 0.f);
+            for (int32_t tri = 0; tri < nt; ++tri) {
+                const int32_t *t = d->mesh + 4 * (int64_t)tri;
+                const float *pa = d->points + 4 * (int64_t)t[0];
+                const float *pb = d->points + 4 * (int64_t)t[1];
+                const float *pc = d->points + 4 * (int64_t)t[2];
+                float *r = &trecs[12 * (int64_t)tri];
+                r[0] = pa[0]; r[1] = pa[1]; r[2] = pa[2];
+                std::memcpy(&r[3], &tri, 4);
+                r[4] = pb[0] - pa[0]; r[5] = pb[1] - pa[1]; r[6] = pb[2] - pa[2]; r[7] = 0.f;
+                r[8] = pc[0] - pa[0]; r[9] = pc[1] - pa[1]; r[10] = pc[2] - pa[2]; r[11] = 0.f;
+            }
+            upload(s->tri_recs, trecs.data(), trecs.size(), total);
+            upload(s->leaf_ids, leaf_ids.data(), leaf_ids.size(), total);
+        }
         upload(s->points, d->points, 4 * (size_t)np, total);
         upload(s->normals, d->normals, 4 * (size_t)d->n_normals, total);
         upload(s->texcoords, d->texcoords, 2 * (size_t)d->n_texcoords, total);
@@ -178,6 +199,8 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.nodes = s->nodes.as<float4>();
         ds.n_nodes = (int32_t)nn;
         ds.tris = s->tris.as<float4>();
+        ds.tri_recs = s->tri_recs.as<float4>();
+        ds.leaf_ids = s->leaf_ids.as<int32_t>();
         ds.points = s->points.as<float4>();
         ds.normals = s->normals.as<float4>();
         ds.n_normals = (int32_t)d->n_normals;
@@ -195,18 +218,18 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.arith_ocl = 0;
         ds.refill = 48;
         ds.node_steps = 24;
-        ds.fetch = 29;
+        ds.fetch = 157;
         ds.walk_min = 0;
 
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device),
                   "hipDeviceGetAttribute");
         s->n_cus = std::max(1, cus);
-        // Five resident blocks of the cast per CU (20 waves) run the 1080p
+        // Six resident blocks of the cast per CU (24 waves) run the 1080p
         // render faster than the eight that fit with the default form (node
-        // pairs, a queue of 8 leaves): fewer rays in flight share the L1 and
-        // L2 better.
-        s->cast_grid = std::max(1, cus * std::max(1, std::min(5, hpt::occupancy_blocks_cast())));
+        // pairs, a queue of 8 leaves, per-triangle records): fewer rays in
+        // flight share the L1 and L2 better.
+        s->cast_grid = std::max(1, cus * std::max(1, std::min(6, hpt::occupancy_blocks_cast())));
         s->shade_grid = std::max(1, cus * std::max(1, hpt::occupancy_blocks_shade()));
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
         *out = s.release();

@@ -175,8 +175,8 @@ struct KernelTimer {
 struct hpt_scene {
     int device = 0;
     hpt::DevScene dev{};
-    hpt::DevMem nodes, tris, points, normals, texcoords, mesh, materials, lights, textures,
-        counters;
+    hpt::DevMem nodes, tris, tri_recs, leaf_ids, points, normals, texcoords, mesh, materials,
+        lights, textures, counters;
     int64_t device_bytes = 0;
     int64_t n_nodes = 0, n_entries = 0, n_triangles = 0, n_textures = 0;
     int cast_grid = 0, shade_grid = 0, n_cus = 1;

@@ -217,12 +217,20 @@ struct LeafQueue {
     int cnt[Q + 1][kBlock];
 };
 
+// The record of leaf entry `at`: in leaf order (direct form), or through
+// the entry's triangle id in the per-triangle records (IND, indirect form);
+// both hold the same values.
+template <bool IND>
+__device__ __forceinline__ const float4 *tri_record(const DevScene &sc, int at) {
+    return IND ? sc.tri_recs + 3 * (int64_t)sc.leaf_ids[at] : sc.tris + 3 * (int64_t)at;
+}
+
 // One leaf round of the FLAT form (see cast_refill).  A lane's pending work
 // is the triangles of its `nq` queued leaves in queue order, `cnt` in all
 // (0 and 0 if none); best / best_id are folded in place.  Queue slots from
 // nq on hold stale or never-written values and are not read.  All 64 lanes
 // must call it.
-template <int Q>
+template <int Q, bool IND>
 __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
                                                 int lane, int nq, int cnt, V3 sp, V3 d,
                                                 float &best, int &best_id) {
@@ -267,7 +275,7 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
                     if (k >= 0) at = lq.off[q][w + own] + min(k, c - 1);
                 }
             }
-            const float4 *rec = sc.tris + 3 * (int64_t)at;
+            const float4 *rec = tri_record<IND>(sc, at);
             const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
             r = sc.arith_ocl ? tri_hit_ocl(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2))
                              : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
@@ -331,7 +339,7 @@ __device__ __forceinline__ int queue_record(const LeafQueue<Q> &lq, int col, int
 // its running best by the reference's rule (the round's tests come after
 // every earlier one, so a tie keeps the earlier best).  The winner's id is
 // read back from its record.  Same contract as flat_leaf_round.
-template <int Q>
+template <int Q, bool IND>
 __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
                                                  const RaySlots &rs,
                                                  unsigned long long *key, int lane, int nq,
@@ -358,7 +366,7 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
             const int k = j - o_excl;
             const int at = queue_record<Q>(lq, w + own, o_nq, k);
             const float4 o4 = rs.org[w + own], d4 = rs.dir[w + own];
-            const float4 *rec = sc.tris + 3 * (int64_t)at;
+            const float4 *rec = tri_record<IND>(sc, at);
             const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
             const float r = sc.arith_ocl ? tri_hit_ocl(xyz(o4), xyz(d4), xyz(r0), xyz(r1), xyz(r2))
                                          : tri_hit(xyz(o4), xyz(d4), xyz(r0), xyz(r1), xyz(r2));
@@ -376,7 +384,7 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
             if (best < 0 || r < best) {
                 best = r;
                 const int at = queue_record<Q>(lq, threadIdx.x, nq, (int)(unsigned)kv);
-                best_id = __float_as_int(sc.tris[3 * (int64_t)at].w);
+                best_id = IND ? sc.leaf_ids[at] : __float_as_int(sc.tris[3 * (int64_t)at].w);
             }
         }
     }
@@ -413,7 +421,9 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
 //         largest count, and lanes keep walking while their queue fills.
 //  KEYED  (FLAT only) the leaf round folds through 64-bit keys in LDS
 //         (keyed_leaf_round) and testers read rays from LDS.
-template <bool PATHS, bool FAST, bool PAIRS, bool PREF, int Q, bool KEYED>
+//  IND    (FLAT only) leaf rounds read the records once per triangle
+//         through the leaf entries' triangle ids (tri_record).
+template <bool PATHS, bool FAST, bool PAIRS, bool PREF, int Q, bool KEYED, bool IND>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
                                             const float *dir3, const float4 *org4,
                                             const float4 *dir4, int64_t n, WorkCounter *wc,
@@ -577,10 +587,10 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             __builtin_amdgcn_wave_barrier();
             const bool own = ray >= 0 && pending > 0;
             if (KEYED)
-                keyed_leaf_round<QN>(sc, lq, rs, s_key, lane, own ? pending : 0,
+                keyed_leaf_round<QN, IND>(sc, lq, rs, s_key, lane, own ? pending : 0,
                                      own ? pcnt : 0, best, best_id);
             else
-                flat_leaf_round<QN>(sc, lq, lane, own ? pending : 0, own ? pcnt : 0, sp, d,
+                flat_leaf_round<QN, IND>(sc, lq, lane, own ? pending : 0, own ? pcnt : 0, sp, d,
                                     best, best_id);
             if (own) {
                 leaves += pending;
@@ -620,7 +630,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
 // bits: 1 node pairs, 2 triangle prefetch, 4 FLAT, and with 4 the leaf
 // queue depth 1 << ((FETCH >> 3) & 3); bit 32 compiles the form for six
 // waves per SIMD (80 VGPRs) instead of eight (64); bit 64 selects the keyed
-// leaf-round fold.
+// leaf-round fold, bit 128 the indirect (per-triangle) records.
 // The keyed forms hold 21 KB (queue of 4) or 29 KB (queue of 8) of LDS per
 // block, so at most 7 or 5 blocks fit in a CU's 160 KB.
 constexpr int cast_waves_per_eu(int fetch) {
@@ -635,8 +645,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     CountRef n_dev, int64_t n_host, WorkCounter *wc, int *out_id, float *out_t) {
     const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
     constexpr int Q = (FETCH & 4) ? (1 << ((FETCH >> 3) & 3)) : 0;
-    cast_refill<PATHS, FAST, (FETCH & 1) != 0, (FETCH & 6) == 2, Q, (FETCH & 64) != 0>(
-        sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
+    cast_refill<PATHS, FAST, (FETCH & 1) != 0, (FETCH & 6) == 2, Q, (FETCH & 64) != 0,
+                (FETCH & 128) != 0>(sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
     wave_done(wc);
 }
 
@@ -1158,6 +1168,12 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
     case 92: HPT_REFILL(P, F, 92); break;                                                        \
     case 116: HPT_REFILL(P, F, 116); break;                                                      \
     case 124: HPT_REFILL(P, F, 124); break;                                                      \
+    case 148: HPT_REFILL(P, F, 148); break;                                                      \
+    case 149: HPT_REFILL(P, F, 149); break;                                                      \
+    case 156: HPT_REFILL(P, F, 156); break;                                                      \
+    case 157: HPT_REFILL(P, F, 157); break;                                                      \
+    case 220: HPT_REFILL(P, F, 220); break;                                                      \
+    case 252: HPT_REFILL(P, F, 252); break;                                                      \
     default: HPT_REFILL(P, F, 0); break;                                                         \
     }
         if (paths) { if (fast) { HPT_REFILL_M(true, true) } else { HPT_REFILL_M(true, false) } }

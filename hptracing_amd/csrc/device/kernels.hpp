@@ -9,6 +9,8 @@
 //            b = offset of its first triangle record (count 0 = empty leaf).
 //   tris   : 48 bytes per leaf entry, in leaf-list order: float4 {pa, id},
 //            float4 {e1 = pb - pa, 0}, float4 {e2 = pc - pa, 0}.
+//   tri_recs, leaf_ids : the same records once per triangle, and the leaf
+//            lists as triangle ids (the indirect fetch form).
 //   shading arrays keep the reference's layouts (float4 points/normals,
 //   float2 texcoords, int4 mesh and lights, 96-byte materials, RGBA8 texels).
 //
@@ -30,6 +32,8 @@ struct DevScene {
     const float4 *nodes;
     int32_t n_nodes;
     const float4 *tris;
+    const float4 *tri_recs;   // the records once per triangle (indirect form)
+    const int32_t *leaf_ids;  // triangle id per leaf entry (indirect form)
     const float4 *points;
     const float4 *normals;
     int32_t n_normals;
@@ -51,7 +55,8 @@ struct DevScene {
     int32_t fetch;      // traversal 2: bit 0 paired node fetch, bit 1 triangle prefetch,
                         // bit 2 leaf tests spread over the wave (not with bit 1),
                         // bits 3-4 (with bit 2) leaf queue depth 1 << value,
-                        // bit 5 six-wave register budget, bit 6 keyed leaf fold
+                        // bit 5 six-wave register budget, bit 6 keyed leaf fold,
+                        // bit 7 per-triangle records through leaf ids
 };
 
 struct Paths {

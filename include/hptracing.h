@@ -147,7 +147,7 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
  * hpt_scene_create (fast_division = 1, traversal = 2, refill_threshold = 48,
- * node_steps = 24, fetch = 29, walk_min = 0, cast grid = min(5, occupancy)
+ * node_steps = 24, fetch = 157, walk_min = 0, cast grid = min(6, occupancy)
  * x CUs, shade grid = occupancy x CUs). */
 typedef struct {
     int32_t fast_division;       /* slab quotients via certified reciprocals */
@@ -169,9 +169,13 @@ typedef struct {
                                     8 and without bit 0, bit 5 builds the
                                     kernel for six waves per SIMD and bit 6
                                     folds each leaf round through 64-bit
-                                    (distance, position) keys in LDS.  The
-                                    compiled forms: 0-5, 12, 13, 20, 21,
-                                    28, 29, 52, 60, 84, 92, 116, 124        */
+                                    (distance, position) keys in LDS; with
+                                    bit 2, bit 7 reads the triangle records
+                                    once per triangle through the leaf
+                                    entries' ids instead of once per leaf
+                                    entry.  The compiled forms: 0-5, 12,
+                                    13, 20, 21, 28, 29, 52, 60, 84, 92,
+                                    116, 124, 148, 149, 156, 157, 220, 252 */
     int32_t walk_min;            /* traversal 2: a round's node phase ends
                                     once fewer lanes than this (0..64) can
                                     walk (0: only when none can)            */

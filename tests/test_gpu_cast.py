@@ -27,7 +27,9 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                         (0, 2, 1, 1, 20), (1, 2, 32, 8, 29), (1, 2, 32, 16, 20),
                         (1, 2, 32, 32, 20, 32), (0, 2, 8, 64, 28, 64), (1, 2, 16, 8, 0, 48),
                         (1, 2, 48, 32, 52), (0, 2, 8, 64, 60), (1, 2, 48, 32, 84),
-                        (0, 2, 1, 1, 116), (1, 2, 16, 8, 124), (1, 2, 32, 16, 92, 16)],
+                        (0, 2, 1, 1, 116), (1, 2, 16, 8, 124), (1, 2, 32, 16, 92, 16),
+                        (1, 2, 48, 24, 157), (0, 2, 1, 1, 156), (1, 2, 16, 16, 148),
+                        (1, 2, 48, 32, 149), (1, 2, 48, 32, 252), (0, 2, 8, 64, 220, 16)],
                 ids=["ieee-refill", "rcp-refill-every-step", "ieee-refill-64", "rcp-phases",
                      "ieee-phases", "rcp-interleaved", "ieee-interleaved", "rcp-refill-pairs",
                      "rcp-refill-prefetch", "ieee-refill-pairs-prefetch",
@@ -37,7 +39,10 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                      "rcp-refill-32-flat-q4", "rcp-flat-q4-walk32", "ieee-flat-q8-walk64",
                      "rcp-refill-walk48", "rcp-flat-q4-wide", "ieee-flat-q8-wide",
                      "rcp-flat-q4-keyed", "ieee-flat-q4-wide-keyed-every-step",
-                     "rcp-flat-q8-wide-keyed", "rcp-flat-q8-keyed-walk16"])
+                     "rcp-flat-q8-wide-keyed", "rcp-flat-q8-keyed-walk16",
+                     "rcp-flat-q8-pairs-indirect", "ieee-flat-q8-indirect-every-step",
+                     "rcp-flat-q4-indirect", "rcp-flat-q4-pairs-indirect",
+                     "rcp-flat-q8-wide-keyed-indirect", "ieee-flat-q8-keyed-indirect-walk16"])
 def fast(request):
     """(fast_division, traversal, refill_threshold, node_steps, fetch[,
     walk_min]) tunings: every one must give the same bits."""
