@@ -1172,6 +1172,7 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
     case 149: HPT_REFILL(P, F, 149); break;                                                      \
     case 156: HPT_REFILL(P, F, 156); break;                                                      \
     case 157: HPT_REFILL(P, F, 157); break;                                                      \
+    case 189: HPT_REFILL(P, F, 189); break;                                                      \
     case 220: HPT_REFILL(P, F, 220); break;                                                      \
     case 252: HPT_REFILL(P, F, 252); break;                                                      \
     default: HPT_REFILL(P, F, 0); break;                                                         \

@@ -37,14 +37,15 @@ int occupancy_blocks_shade();
 // triangle prefetch (2), wave-spread leaf rounds (4) with a leaf queue of
 // depth 1, 2, 4 or 8 (bits 3-4 = 0..3; with 4 only, and without 2); with
 // a queue of 4 or 8, bit 5 (32) compiles the form for six waves per SIMD
-// and bit 6 (64) folds the leaf rounds through keys in LDS; bit 7 (128)
+// and (without node pairs) bit 6 (64) folds the leaf rounds through keys in
+// LDS; bit 7 (128)
 // reads the triangle records once per triangle through the leaf entries'
 // ids.  These are the compiled forms.
 inline bool valid_fetch(int f) {
     switch (f) {
     case 0: case 1: case 2: case 3: case 4: case 5: case 12: case 13: case 20: case 21:
     case 28: case 29: case 52: case 60: case 84: case 92: case 116: case 124: case 148:
-    case 149: case 156: case 157: case 220: case 252:
+    case 149: case 156: case 157: case 189: case 220: case 252:
         return true;
     default:
         return false;

@@ -166,8 +166,8 @@ typedef struct {
                                     wave (not with bit 1), and with bit 2,
                                     bits 3-4 = q let a lane queue 1 << q
                                     leaves per round; with a queue of 4 or
-                                    8 and without bit 0, bit 5 builds the
-                                    kernel for six waves per SIMD and bit 6
+                                    8, bit 5 builds the kernel for six
+                                    waves per SIMD and (without bit 0) bit 6
                                     folds each leaf round through 64-bit
                                     (distance, position) keys in LDS; with
                                     bit 2, bit 7 reads the triangle records
@@ -175,7 +175,8 @@ typedef struct {
                                     entries' ids instead of once per leaf
                                     entry.  The compiled forms: 0-5, 12,
                                     13, 20, 21, 28, 29, 52, 60, 84, 92,
-                                    116, 124, 148, 149, 156, 157, 220, 252 */
+                                    116, 124, 148, 149, 156, 157, 189, 220,
+                                    252                                     */
     int32_t walk_min;            /* traversal 2: a round's node phase ends
                                     once fewer lanes than this (0..64) can
                                     walk (0: only when none can)            */

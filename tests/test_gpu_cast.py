@@ -29,7 +29,8 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                         (1, 2, 48, 32, 52), (0, 2, 8, 64, 60), (1, 2, 48, 32, 84),
                         (0, 2, 1, 1, 116), (1, 2, 16, 8, 124), (1, 2, 32, 16, 92, 16),
                         (1, 2, 48, 24, 157), (0, 2, 1, 1, 156), (1, 2, 16, 16, 148),
-                        (1, 2, 48, 32, 149), (1, 2, 48, 32, 252), (0, 2, 8, 64, 220, 16)],
+                        (1, 2, 48, 32, 149), (1, 2, 48, 32, 252), (0, 2, 8, 64, 220, 16),
+                        (1, 2, 48, 24, 189), (0, 2, 1, 1, 189)],
                 ids=["ieee-refill", "rcp-refill-every-step", "ieee-refill-64", "rcp-phases",
                      "ieee-phases", "rcp-interleaved", "ieee-interleaved", "rcp-refill-pairs",
                      "rcp-refill-prefetch", "ieee-refill-pairs-prefetch",
@@ -42,7 +43,9 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                      "rcp-flat-q8-wide-keyed", "rcp-flat-q8-keyed-walk16",
                      "rcp-flat-q8-pairs-indirect", "ieee-flat-q8-indirect-every-step",
                      "rcp-flat-q4-indirect", "rcp-flat-q4-pairs-indirect",
-                     "rcp-flat-q8-wide-keyed-indirect", "ieee-flat-q8-keyed-indirect-walk16"])
+                     "rcp-flat-q8-wide-keyed-indirect", "ieee-flat-q8-keyed-indirect-walk16",
+                     "rcp-flat-q8-pairs-wide-indirect",
+                     "ieee-flat-q8-pairs-wide-indirect-every-step"])
 def fast(request):
     """(fast_division, traversal, refill_threshold, node_steps, fetch[,
     walk_min]) tunings: every one must give the same bits."""
