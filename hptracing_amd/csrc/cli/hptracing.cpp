@@ -111,15 +111,18 @@ int usage(const char *argv0) {
                  "usage: %s -i scene.obj -o out.png [--mtl dir/] [--width 500] [--height 500]\n"
                  "  [--view 0,0,-500] [--up 0,1,0] [--right 1,0,0] [--angle 1.0 | -1 for VR]\n"
                  "  [-x] [-s 10] [-d 6] [--brightness 1] [--no-diffuse] [--background 0,0,0]\n"
-                 "  [--seed N]\n", argv0);
+                 "  [--seed N] [--progressive PREFIX]\n"
+                 "  --progressive writes PREFIX<passes>.png each time the render server\n"
+                 "  publishes more sample passes (the refining frames the original tool's\n"
+                 "  window shows); -o is then optional\n", argv0);
     return 2;
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
-    std::string input, mtl, output, view = "0,0,-500", up = "0,1,0", right = "1,0,0",
-                                     background = "0,0,0";
+    std::string input, mtl, output, progressive, view = "0,0,-500", up = "0,1,0",
+                                                 right = "1,0,0", background = "0,0,0";
     int width = 500, height = 500, sample = 10, depth = 6;
     float angle = 1.0f, brightness = 1.0f;
     bool supersample = false, no_diffuse = false;
@@ -137,6 +140,7 @@ int main(int argc, char **argv) {
         if (a == "-i" || a == "--input") next(input);
         else if (a == "--mtl") next(mtl);
         else if (a == "-o" || a == "--output") next(output);
+        else if (a == "--progressive") next(progressive);
         else if (a == "--view") next(view);
         else if (a == "--up") next(up);
         else if (a == "--right") next(right);
@@ -157,9 +161,9 @@ int main(int argc, char **argv) {
         }
     }
     if (input.empty()) return usage(argv[0]);
-    if (output.empty()) {
+    if (output.empty() && progressive.empty()) {
         std::fprintf(stderr, "no -o given: the interactive window of the original tool is not "
-                             "part of this build\n");
+                             "part of this build (--progressive writes its refining frames)\n");
         return 2;
     }
     if (mtl.empty()) {  // hptracing.cpp:238-244
@@ -197,26 +201,63 @@ int main(int argc, char **argv) {
     v.brightness = brightness;
     v.seed = seed;
     std::vector<uint8_t> rgba((size_t)width * height * 4);
-    hpt_render_stats st;
-    if (hpt_render(scene, &v, rgba.data(), nullptr, &st) != HPT_OK) {
-        std::fprintf(stderr, "render: %s\n", hpt_last_error());
-        return 1;
+    // hptracing.cpp:285-295: the image is written flipped in y.
+    auto save = [&](const std::string &path) {
+        std::vector<uint8_t> rgb((size_t)width * height * 3);
+        for (int y = 0; y < height; ++y)
+            for (int x = 0; x < width; ++x)
+                for (int k = 0; k < 3; ++k)
+                    rgb[((size_t)(height - 1 - y) * width + x) * 3 + k] =
+                        rgba[((size_t)y * width + x) * 4 + k];
+        const bool png = path.size() > 4 && path.substr(path.size() - 4) == ".png";
+        const bool ok = png ? write_png(path, rgb, width, height) : write_ppm(path, rgb, width, height);
+        if (!ok) std::fprintf(stderr, "cannot write %s\n", path.c_str());
+        return ok;
+    };
+    bool ok = true;
+    if (progressive.empty()) {
+        hpt_render_stats st;
+        if (hpt_render(scene, &v, rgba.data(), nullptr, &st) != HPT_OK) {
+            std::fprintf(stderr, "render: %s\n", hpt_last_error());
+            return 1;
+        }
+        std::fprintf(stderr, "Render done: %.3fs, %lld rays, %.1f Mrays/s\n", st.seconds,
+                     (long long)st.rays_cast, st.rays_cast / st.seconds / 1e6);
+        ok = save(output);
+    } else {
+        // The view as one progressive request of a render server whose
+        // arena holds it (about 1.3 KB per camera ray).
+        const int64_t rays = (int64_t)width * height * (supersample ? 4 : 1);
+        hpt_server_opts opts{rays * 2048 + ((int64_t)64 << 20)};
+        hpt_server *srv = nullptr;
+        int64_t id = 0;
+        if (hpt_server_create(scene, &opts, &srv) != HPT_OK ||
+            hpt_server_submit_ex(srv, &v, nullptr, nullptr, HPT_SERVE_PROGRESSIVE, &id) != HPT_OK) {
+            std::fprintf(stderr, "server: %s\n", hpt_last_error());
+            return 1;
+        }
+        int32_t passes = 0;
+        while (ok && passes < sample) {
+            int32_t got = 0;
+            if (hpt_server_peek(srv, id, passes + 1, -1.0, rgba.data(), &got) != HPT_OK) {
+                std::fprintf(stderr, "server: %s\n", hpt_last_error());
+                return 1;
+            }
+            passes = got;
+            ok = save(progressive + std::to_string(passes) + ".png");
+            std::fprintf(stderr, "%d of %d samples\n", passes, sample);
+        }
+        hpt_request_info info;
+        const int rc = hpt_server_wait(srv, id, -1.0, &info);
+        hpt_server_destroy(srv);
+        if (rc != HPT_OK) {
+            std::fprintf(stderr, "server: %s\n", hpt_last_error());
+            return 1;
+        }
+        std::fprintf(stderr, "Render done: %.3fs\n", info.latency_ms / 1e3);
+        if (ok && !output.empty()) ok = save(output);
     }
-    std::fprintf(stderr, "Render done: %.3fs, %lld rays, %.1f Mrays/s\n", st.seconds,
-                 (long long)st.rays_cast, st.rays_cast / st.seconds / 1e6);
-    std::vector<uint8_t> rgb((size_t)width * height * 3);
-    for (int y = 0; y < height; ++y)
-        for (int x = 0; x < width; ++x)
-            for (int k = 0; k < 3; ++k)
-                rgb[((size_t)(height - 1 - y) * width + x) * 3 + k] =
-                    rgba[((size_t)y * width + x) * 4 + k];
-    const bool png = output.size() > 4 && output.substr(output.size() - 4) == ".png";
-    const bool ok = png ? write_png(output, rgb, width, height) : write_ppm(output, rgb, width, height);
     hpt_scene_destroy(scene);
     hpt_host_free(host);
-    if (!ok) {
-        std::fprintf(stderr, "cannot write %s\n", output.c_str());
-        return 1;
-    }
-    return 0;
+    return ok ? 0 : 1;
 }

@@ -70,3 +70,28 @@ def test_cli_errors(tmp_path):
     assert res.returncode == 1 and "missing.obj" in res.stderr
     res = subprocess.run([CLI, "-i", "scene.obj"], capture_output=True, text=True, timeout=60)
     assert res.returncode == 2
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+def test_cli_progressive_frames(tmp_path, room):
+    """--progressive writes one file per published pass count; the frame of S
+    passes is hpt_render's image of the same view with S samples."""
+    from hptracing_amd.runner import TraceRunner, make_view
+    obj = scenes.write_room(str(tmp_path / "scene"), 0)
+    prefix = str(tmp_path / "frame_")
+    out = str(tmp_path / "final.png")
+    cmd = [CLI, "-i", obj, "-o", out, "--progressive", prefix, "--width", "32", "--height",
+           "20", "-s", "20", "-d", "4", "--view", "5,5,-6", "--right", "-1,0,0", "--up",
+           "0,1,0", "--seed", "3"]
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    frames = sorted(int(f[len("frame_"):-4]) for f in os.listdir(tmp_path)
+                    if f.startswith("frame_"))
+    assert frames and frames[-1] == 20 and frames == sorted(set(frames))
+    r = TraceRunner(room)
+    for s in frames:
+        v = make_view((5, 5, -6), (0, 1, 0), (-1, 0, 0), 32 / 20, 1.0, 32, 20, 1, 1,
+                      (0, 0, 0), s, 4, 0, 1.0, seed=3)
+        img = read_png(f"{prefix}{s}.png")
+        assert np.array_equal(img, r.render(v)[::-1, :, :3]), s
+    assert np.array_equal(read_png(out), read_png(f"{prefix}20.png"))
