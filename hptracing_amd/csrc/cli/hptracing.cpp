@@ -243,6 +243,7 @@ int main(int argc, char **argv) {
                 std::fprintf(stderr, "server: %s\n", hpt_last_error());
                 return 1;
             }
+            if (got <= passes) break;  // the request ended
             passes = got;
             ok = save(progressive + std::to_string(passes) + ".png");
             std::fprintf(stderr, "%d of %d samples\n", passes, sample);

@@ -138,7 +138,7 @@ class PreviewSession:
                 if passes <= seen:
                     if timeout_s >= 0:
                         raise TimeoutError("no new passes before the timeout")
-                    continue
+                    break  # without a timeout peek only returns early once the request ended
                 seen = passes
                 yield img, passes
         finally:
