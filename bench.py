@@ -288,6 +288,16 @@ def bench_serving(args):
     log(f"[bench] serving scene on the device: {scene_bytes / 1e9:.2f} GB")
     views = serving.request_views(args.serve_requests, seed=3, depth=args.depth)
     runner.render(views[0])  # warm
+    # The launch forms are tuned on this scene's requests (the library's
+    # defaults are tuned on the 1080p scene): hpt_autotune on a 1280x720
+    # view, then on a panorama, of the mix.  Both the served and the
+    # one-at-a-time runs below use the result.
+    t1 = time.time()
+    tuned = runner.autotune(next(v for v in views if v.height > 0 and v.sample_x >= 1280))
+    pano = [v for v in views if v.height < 0]
+    if pano:
+        tuned = runner.autotune(pano[0])
+    log(f"[bench] serving scene autotuned in {time.time() - t1:.1f}s: {tuned}")
     seq_s, seq_rays = serving.one_at_a_time(runner, views)
     seq_mrays = seq_rays / seq_s / 1e6
     log(f"[bench] one at a time: {len(views)} requests {seq_s:.2f}s {seq_mrays:.1f} Mrays/s")
@@ -305,7 +315,7 @@ def bench_serving(args):
     log(f"[bench] served all at once: {burst.wall_s:.2f}s {burst.mrays_s:.1f} Mrays/s, "
         f"p99 {burst.p99_ms:.1f} ms")
     return {"scene_triangles": kd.n_triangles, "scene_device_bytes": scene_bytes,
-            "requests": len(views),
+            "requests": len(views), "tuning": tuned,
             "one_at_a_time_s": round(seq_s, 3), "one_at_a_time_mrays_s": round(seq_mrays, 2),
             "served_burst_s": round(burst.wall_s, 3),
             "served_burst_mrays_s": round(burst.mrays_s, 2),

@@ -329,8 +329,9 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
             hpt_tuning t = start;
             set(t, v);
             if (!valid_fetch(t.fetch)) continue;
+            // A knob moves only for a gain above the run-to-run noise.
             const double ms = timed(t, true);
-            if (ms < best_ms) {
+            if (ms < best_ms * 0.99) {
                 best_ms = ms;
                 best = t;
             }
@@ -353,13 +354,16 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
     }
     if (shade) {
         const int socc = std::max(1, base.shade_blocks_per_cu);
-        double best_shade = 1e30;
+        hpt_tuning t0 = best;
+        t0.shade_blocks_per_cu = socc;
+        double best_shade = timed(t0, false);
         int best_sbpc = socc;
-        for (int sb : {std::max(1, socc / 2), socc, std::min(64, socc * 2)}) {
+        for (int sb : {std::max(1, socc / 2), std::min(64, socc * 2)}) {
+            if (sb == socc) continue;
             hpt_tuning t = best;
             t.shade_blocks_per_cu = sb;
             const double ms = timed(t, false);
-            if (ms < best_shade) {
+            if (ms < best_shade * 0.99) {
                 best_shade = ms;
                 best_sbpc = sb;
             }
@@ -396,12 +400,15 @@ int hpt_autotune(hpt_scene *s, const hpt_view *view, int max_rays, hpt_tuning *c
     return guarded([&] {
         if (!s || !view) fail(HPT_ERR_INVALID, "null argument");
         validate_view(view);
-        // A reduced copy of the view: same camera and depth, one pass, the
-        // image shrunk until it has at most max_rays camera rays.
+        // A reduced copy of the view: same camera and depth, the passes of
+        // one launch group (a render casts up to kPassSlices passes per
+        // launch, and the fill and tail of a launch depend on its size), the
+        // image shrunk until camera rays x passes is at most max_rays.
         hpt_view v = *view;
-        v.sample = 1;
-        if (max_rays <= 0) max_rays = 1 << 23;
-        while ((int64_t)v.sample_x * v.supersample_x * v.sample_y * v.supersample_y > max_rays &&
+        v.sample = std::min(view->sample, kPassSlices);
+        if (max_rays <= 0) max_rays = 1 << 24;
+        while ((int64_t)v.sample_x * v.supersample_x * v.sample_y * v.supersample_y * v.sample >
+                   max_rays &&
                v.sample_x > 8 && v.sample_y > 8) {
             v.sample_x /= 2;
             v.sample_y /= 2;

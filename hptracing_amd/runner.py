@@ -66,8 +66,9 @@ class TraceRunner:
         N.check_device(N.device().hpt_scene_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, v: N.View, max_rays: int = 0) -> dict:
-        """Pick the fastest tuning on one-pass renders of `v` (shrunk to at
-        most max_rays camera rays; 0: the library's default, 2^23)."""
+        """Pick the fastest tuning on renders of `v` with one launch group's
+        passes (min(sample, 16)), shrunk to at most max_rays camera rays x
+        passes (0: the library's default, 2^24)."""
         t = N.Tuning()
         N.check_device(N.device().hpt_autotune(self._h, C.byref(v), max_rays, C.byref(t)))
         return {k: getattr(t, k) for k, _ in N.Tuning._fields_}

@@ -277,11 +277,13 @@ int hpt_render(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
 int hpt_render_ex(hpt_scene *scene, const hpt_view *view, uint8_t *rgba,
                   float *accum, hpt_render_stats *stats, unsigned flags);
 
-/* Tune the scene's launch forms (hpt_tuning) on one-pass renders of `view`
- * shrunk to at most max_rays camera rays (<= 0: 2^23): coordinate descent
+/* Tune the scene's launch forms (hpt_tuning) on renders of `view` with the
+ * passes of one launch group (min(sample, 16)), the image shrunk until
+ * camera rays x passes is at most max_rays (<= 0: 2^24): coordinate descent
  * over traversal, fetch form, refill threshold, node steps, division form
  * and cast grid (two sweeps, each knob taking the value with the shortest
- * cast time while the others are held), then the shade grid.  Keeps the
+ * cast time, if more than 1 % shorter, while the others are held), then the
+ * shade grid.  Keeps the
  * result on the scene and returns it in `chosen` (may be NULL).  No
  * reference counterpart. */
 int hpt_autotune(hpt_scene *scene, const hpt_view *view, int max_rays,
