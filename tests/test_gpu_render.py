@@ -58,6 +58,44 @@ def test_render_supersample_vr(room, oracle_mod):
     assert frac > 0.99, frac
 
 
+@pytest.fixture(scope="module")
+def full_small(scene_dir):
+    """The 1080p scene's parts at a reduced triangle count: the room, the
+    water grid (refraction, IOR 1.33), the textured surface of revolution
+    (texture fetch) and the mirror sphere (specular)."""
+    import os
+    from hptracing_amd.scene import KDTree
+    p = scenes.write_full(os.path.join(scene_dir, "full_small"), 0, water_side=60, rev=(60, 40),
+                          sphere=(24, 24))
+    return KDTree(p, os.path.dirname(p) + "/")
+
+
+def test_render_full_scene_matches_oracle_paths(full_small, oracle_mod):
+    """Every material kind of the 1080p scene (texture fetch, specular,
+    refraction, diffuse, light) in the 1080p view with -x and depth 8,
+    sample for sample against the oracle."""
+    from hptracing_amd.runner import TraceRunner
+    kd = full_small
+    # The view sees textured, mirror and refracting surfaces.
+    W, H = 64, 36
+    ys, xs = np.meshgrid((np.arange(H) + 0.5) / H - 0.5, (np.arange(W) + 0.5) / W - 0.5,
+                         indexing="ij")
+    d = np.stack([-xs.ravel() * (W / H), ys.ravel(), np.ones(W * H)], 1).astype(np.float32)
+    o = np.tile(np.array(scenes.ROOM_VIEW["view_p"], np.float32), (W * H, 1))
+    ids, _ = TraceRunner(kd).cast(o, d)
+    ids = ids.cpu().numpy()
+    m = kd.materials[kd.geometries[ids[ids >= 0], 3]]
+    assert (m["diffuse_texture_id"] >= 0).mean() > 0.005
+    assert (m["specular_possibility"] > 0.5).mean() > 0.01
+    assert (m["refract_possibility"] > 0.5).mean() > 0.01
+    frac, rgba, ref_rgba, live, ref_live, *_ = _compare(
+        kd, oracle_mod, width=W / H, sample_x=W, sample_y=H, supersample_x=2, supersample_y=2,
+        sample=8, depth=8, background_color=(0.0, 0.0, 0.0))
+    assert frac > 0.99, frac
+    assert np.abs(live - ref_live).sum() <= 0.01 * ref_live.sum()
+    assert (np.abs(rgba.astype(int) - ref_rgba.astype(int)) <= 1).mean() > 0.99
+
+
 def test_render_deterministic(room):
     from hptracing_amd.runner import TraceRunner, make_view
     r = TraceRunner(room)
