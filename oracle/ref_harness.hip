@@ -1,12 +1,17 @@
-// Launcher of the reference's own kdtree_intersect (compiled from kernel.cl
-// by oracle/build_ref.sh) -- TEST INFRASTRUCTURE.
+// Launchers of the reference's own kernels (compiled from kernel.cl by
+// oracle/build_ref.sh) -- TEST INFRASTRUCTURE.
 //
-// Reproduces the first intersect of TraceRunner::run (trace_runner.cpp:
-// 192-206): every ray is a 144-byte unit_data record (types.h.cl:8-17) with
-// orig_id = index, strength = 1, start_p and in_dir set; v_s0 = 0..n-1;
-// sample_n = 1 and a black background; 256-wide groups over
-// (n/256 + 1)*256 items.  Returns the geometry int4 and intersect_number the
-// kernel writes into each record (left at -1 / -42 on a miss).
+// ref_kdtree_intersect reproduces the first intersect of TraceRunner::run
+// (trace_runner.cpp:192-206): every ray is a 144-byte unit_data record
+// (types.h.cl:8-17) with orig_id = index, strength = 1, start_p and in_dir
+// set; v_s0 = 0..n-1; sample_n = 1 and a black background; 256-wide groups
+// over (n/256 + 1)*256 items.  Returns the geometry int4 and
+// intersect_number the kernel writes into each record (left at -1 / -42 on
+// a miss).
+//
+// ref_render reproduces the whole of TraceRunner::run up to results_mem
+// (trace_runner.cpp:116-327) with the reference's kernels and its host
+// protocol for the list sizes.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -123,6 +128,227 @@ int ref_kdtree_intersect_timed(const char *co_path, const float *points, int64_t
             std::memcpy(geometry + 4 * (size_t)i, data[i].geometry, 16);
             t[i] = data[i].intersect_number;
         }
+        rc = 0;
+    } while (false);
+    for (void *p : keep) (void)hipFree(p);
+    if (mod) (void)hipModuleUnload(mod);
+    return rc;
+}
+
+// TraceRunner::run (trace_runner.cpp:116-327) up to results_mem, with the
+// reference's kernels: set_viewdirs[_vr] over (W*ssx, H*ssy), the first
+// kdtree_intersect with sample_n = sample whose S1 list is kept, then per
+// sample the restore of the records and of S1 and `depth` levels of
+// kdtree_intersect (skipped while S0 is empty), s1_run and the four s2
+// kernels, the host reading v_sizes after each stage and zeroing the
+// consumed counts exactly as run() does.  Differences, none of which the
+// kernels can see: results_mem starts at zero (run() leaves it
+// uninitialised) and the per-slot seeds are the caller's instead of
+// rand().  co_path is the build whose s1_run takes its texture argument as
+// an int (ref_kernels_notex.co: an OpenCL image object cannot be passed
+// through HIP), so every material must be untextured (checked; -2
+// otherwise).
+//
+// Arrays as hpt_scene_desc holds them: points and normals 4 floats each,
+// texcoords 2, mesh and lights int4, materials 96 bytes, nodes 48 bytes.
+// seeds: W*ssx*H*ssy values; results: 3 floats per camera ray; s0_sizes
+// (may be NULL): sample*depth ints, the S0 count each level's intersect
+// ran over.
+int ref_render(const char *co_path, const float *points, int64_t n_points, const float *normals,
+               int64_t n_normals, const float *texcoords, int64_t n_texcoords,
+               const int32_t *mesh, int64_t n_tris, const void *materials, int64_t n_materials,
+               const int32_t *lights, int64_t n_lights, const void *nodes, int64_t n_nodes,
+               const int32_t *leaf, int64_t n_leaf, const float *view, const float *up,
+               const float *right, float width, float height, int32_t sample_x,
+               int32_t sample_y, int32_t ssx, int32_t ssy, const float *background,
+               int32_t sample, int32_t depth, int32_t disable_diffuse, const int64_t *seeds,
+               float *results, int32_t *s0_sizes) {
+    std::vector<void *> keep;
+    hipModule_t mod = nullptr;
+    int rc = -1;
+    do {
+        const uint8_t *mats = static_cast<const uint8_t *>(materials);
+        bool textured = false;
+        for (int64_t m = 0; m < n_materials; ++m) {
+            int32_t diffuse_tex, ambient_tex;  // Material offsets 76 and 80 (types.h.cl)
+            std::memcpy(&diffuse_tex, mats + 96 * m + 76, 4);
+            std::memcpy(&ambient_tex, mats + 96 * m + 80, 4);
+            textured = textured || diffuse_tex >= 0 || ambient_tex >= 0;
+        }
+        if (textured) {
+            g_err = "ref_render: textured materials are not supported";
+            rc = -2;
+            break;
+        }
+        const int32_t gx = sample_x * ssx, gy = sample_y * ssy;
+        const int32_t R = gx * gy;
+        if (sample_x <= 0 || sample_y <= 0 || ssx <= 0 || ssy <= 0 || sample <= 0 || depth <= 0) {
+            g_err = "ref_render: bad view";
+            break;
+        }
+        // set_viewdirs_vr indexes its records with sample_x * ssy as the row
+        // length over a range of sample_x * ssx columns: in bounds only when
+        // the two supersampling factors are equal.
+        if (height < 0 && ssx != ssy) {
+            g_err = "ref_render: the equirectangular camera needs ssx == ssy";
+            break;
+        }
+        if (!ok(hipModuleLoad(&mod, co_path), "hipModuleLoad")) break;
+        const char *names[] = {"set_viewdirs", "set_viewdirs_vr", "kdtree_intersect", "s1_run",
+                               "s2_refract_run", "s2_specular_run", "s2_diffuse_run",
+                               "s2_light_run"};
+        hipFunction_t fn[8];
+        bool got = true;
+        for (int k = 0; k < 8 && got; ++k)
+            got = ok(hipModuleGetFunction(&fn[k], mod, names[k]), names[k]);
+        if (!got) break;
+
+        auto alloc = [&](size_t bytes) -> void * {
+            void *p = nullptr;
+            if (!ok(hipMalloc(&p, bytes < 64 ? 64 : bytes), "hipMalloc")) return nullptr;
+            keep.push_back(p);
+            if (!ok(hipMemset(p, 0, bytes < 64 ? 64 : bytes), "hipMemset")) return nullptr;
+            return p;
+        };
+        auto upload = [&](const void *src, size_t bytes) -> void * {
+            void *p = alloc(bytes);
+            if (p && bytes && !ok(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice), "upload"))
+                return nullptr;
+            return p;
+        };
+        void *d_points = upload(points, 16 * (size_t)n_points);
+        void *d_normals = upload(normals, 16 * (size_t)n_normals);
+        void *d_texcoords = upload(texcoords, 8 * (size_t)n_texcoords);
+        void *d_mesh = upload(mesh, 16 * (size_t)n_tris);
+        void *d_mats = upload(materials, 96 * (size_t)n_materials);
+        void *d_lights = upload(lights, 16 * (size_t)n_lights);
+        void *d_nodes = upload(nodes, 48 * (size_t)n_nodes);
+        void *d_leaf = upload(leaf, 4 * (size_t)n_leaf);
+        int32_t no_texture = 0;  // s1_run's texture argument (an int in this build)
+        void *d_initial = alloc(sizeof(Unit) * (size_t)R);
+        void *d_data = alloc(sizeof(Unit) * (size_t)R);
+        void *d_results = alloc(12 * (size_t)R);
+        void *d_sizes = alloc(40);
+        void *d_seeds = upload(seeds, 8 * (size_t)R);
+        std::vector<int32_t> iota(R);
+        for (int32_t i = 0; i < R; ++i) iota[i] = i;
+        void *d_cache = upload(iota.data(), 4 * (size_t)R);
+        void *d_stage[6];
+        bool all = d_points && d_normals && d_texcoords && d_mesh && d_mats && d_lights &&
+                   d_nodes && d_leaf && d_initial && d_data && d_results && d_sizes &&
+                   d_seeds && d_cache;
+        for (int k = 0; k < 6; ++k) {
+            d_stage[k] = alloc(4 * (size_t)R);
+            all = all && d_stage[k];
+        }
+        if (!all) break;
+
+        // set_viewdirs[_vr] over exactly (gx, gy) items: the kernels test
+        // `> sample_x`, so a larger range would write past the records.
+        int32_t bx = 256;
+        while (gx % bx) bx /= 2;
+        const F3 vp{view[0], view[1], view[2], 0}, tp{up[0], up[1], up[2], 0},
+            rp{right[0], right[1], right[2], 0};
+        if (height < 0) {
+            int32_t shift = (int32_t)right[0] * ssx;  // trace_runner.cpp:132-140
+            int32_t kx = sample_x * ssy, ky = sample_y * ssy;
+            void *a[] = {(void *)&vp, &shift, &kx, &ky, &d_initial};
+            if (!ok(hipModuleLaunchKernel(fn[1], gx / bx, gy, 1, bx, 1, 1, 0, nullptr, a, nullptr),
+                    "set_viewdirs_vr"))
+                break;
+        } else {
+            void *a[] = {(void *)&vp, (void *)&tp, (void *)&rp, &width, &height,
+                         (void *)&gx, (void *)&gy, &d_initial};
+            if (!ok(hipModuleLaunchKernel(fn[0], gx / bx, gy, 1, bx, 1, 1, 0, nullptr, a, nullptr),
+                    "set_viewdirs"))
+                break;
+        }
+        if (!ok(hipDeviceSynchronize(), "set_viewdirs")) break;
+
+        int32_t sizes[10];
+        auto put = [&]() {
+            return ok(hipMemcpy(d_sizes, sizes, 40, hipMemcpyHostToDevice), "write sizes");
+        };
+        auto get = [&]() {
+            return ok(hipMemcpy(sizes, d_sizes, 40, hipMemcpyDeviceToHost), "read sizes");
+        };
+        // ND_RANGE of trace_runner.cpp:183-186: (n/256 + 1) groups of 256.
+        auto nd = [&](hipFunction_t f, int32_t n, void **a, const char *what) {
+            return ok(hipModuleLaunchKernel(f, (unsigned)(n / 256 + 1), 1, 1, 256, 1, 1, 0,
+                                            nullptr, a, nullptr),
+                      what);
+        };
+        const F3 bg{background[0], background[1], background[2], 0};
+        int32_t n_nodes32 = (int32_t)n_nodes, normals_size = (int32_t)n_normals,
+                texcoords_size = (int32_t)n_texcoords, lights_size = (int32_t)n_lights;
+
+        // The first intersect (sample_n = sample) and the cached S1 list.
+        std::memset(sizes, 0, sizeof(sizes));
+        sizes[0] = R;
+        if (!put()) break;
+        {
+            int32_t sample_n = sample;
+            void *a[] = {&d_sizes, &d_initial, &d_cache, &d_stage[1], &d_points, &d_mesh,
+                         &d_leaf, &d_nodes, &n_nodes32, &d_results, (void *)&bg, &sample_n};
+            if (!nd(fn[2], R, a, "kdtree_intersect")) break;
+        }
+        if (!get()) break;
+        if (!ok(hipMemcpy(d_cache, d_stage[1], 4 * (size_t)R, hipMemcpyDeviceToDevice), "cache s1"))
+            break;
+        const int32_t initial_s1 = sizes[1];
+
+        bool fine = true;
+        for (int32_t ii = 0; ii < sample && fine; ++ii) {
+            std::memset(sizes, 0, sizeof(sizes));
+            sizes[1] = initial_s1;
+            fine = put() &&
+                   ok(hipMemcpy(d_data, d_initial, sizeof(Unit) * (size_t)R,
+                                hipMemcpyDeviceToDevice),
+                      "restore records") &&
+                   ok(hipMemcpy(d_stage[1], d_cache, 4 * (size_t)R, hipMemcpyDeviceToDevice),
+                      "restore s1");
+            for (int32_t i = 0; i < depth && fine; ++i) {
+                if (s0_sizes) s0_sizes[(size_t)ii * depth + i] = sizes[0];
+                if (sizes[0]) {
+                    int32_t one = 1;
+                    void *a[] = {&d_sizes, &d_data, &d_stage[0], &d_stage[1], &d_points, &d_mesh,
+                                 &d_leaf, &d_nodes, &n_nodes32, &d_results, (void *)&bg, &one};
+                    fine = nd(fn[2], sizes[0], a, "kdtree_intersect");
+                }
+                fine = fine && ok(hipDeviceSynchronize(), "kdtree_intersect") && get();
+                if (!fine) break;
+                sizes[0] = 0;
+                if (sizes[1]) {
+                    void *a[] = {&d_sizes,     &d_data,       &d_stage[1],     &d_results,
+                                 &d_stage[2],  &d_stage[3],   &d_stage[4],     &d_stage[5],
+                                 &d_points,    &d_normals,    &normals_size,   &d_texcoords,
+                                 &texcoords_size, &d_mats,    &no_texture,     &disable_diffuse,
+                                 &d_seeds};
+                    fine = nd(fn[3], sizes[1], a, "s1_run");
+                }
+                fine = fine && ok(hipDeviceSynchronize(), "s1_run") && get();
+                if (!fine) break;
+                sizes[0] = sizes[1] = 0;
+                if (!(fine = put())) break;
+                if (i == depth - 1) break;
+                void *ar[] = {&d_sizes, &d_data, &d_stage[2], &d_stage[0]};
+                void *as[] = {&d_sizes, &d_data, &d_stage[3], &d_stage[0], &d_seeds};
+                void *ad[] = {&d_sizes, &d_data, &d_stage[4], &d_stage[0], &d_seeds};
+                void *al[] = {&d_sizes,  &d_data,      &d_stage[5], &d_stage[0],
+                              &d_lights, &lights_size, &d_points,   &d_seeds};
+                void **args[4] = {ar, as, ad, al};
+                for (int k = 0; k < 4 && fine; ++k)
+                    if (sizes[k + 2]) fine = nd(fn[4 + k], sizes[k + 2], args[k], names[4 + k]);
+                fine = fine && ok(hipDeviceSynchronize(), "s2") && get();
+                if (!fine) break;
+                sizes[2] = sizes[3] = sizes[4] = sizes[5] = 0;
+                fine = put();
+            }
+        }
+        if (!fine) break;
+        if (!ok(hipMemcpy(results, d_results, 12 * (size_t)R, hipMemcpyDeviceToHost),
+                "download results"))
+            break;
         rc = 0;
     } while (false);
     for (void *p : keep) (void)hipFree(p);

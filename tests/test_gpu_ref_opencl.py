@@ -13,6 +13,7 @@ import os
 import numpy as np
 import pytest
 
+from hptracing_amd import _native as N
 from hptracing_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -29,6 +30,10 @@ needs_ref = pytest.mark.skipif(not (os.path.exists(CO) and os.path.exists(LIB)),
 def ref_cast(kd, org, d, timed=False):
     """The reference kernel's geometry int4 and t per ray (and, with
     `timed`, its launch's device time in ms)."""
+    # torch's HIP runtime must be the first in the process (it reports no
+    # GPU after another copy has opened the device); loading the library
+    # through _native does that before this harness's runtime starts.
+    N.device()
     lib = C.CDLL(LIB)
     vp = C.c_void_p
     lib.ref_kdtree_intersect_timed.argtypes = [C.c_char_p, vp, C.c_int64, vp, C.c_int64, vp,
