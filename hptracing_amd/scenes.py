@@ -267,20 +267,23 @@ def write_room(directory: str, seed: int = 0) -> str:
 
 
 def write_full(directory: str, seed: int = 0, water_side: int = 500, rev=(400, 250),
-               sphere=(100, 100)) -> str:
+               sphere=(100, 100), textured: bool = True) -> str:
     """The full path-trace scene (water_side=500: 500k water triangles;
-    water_side=1000 gives the 2M-triangle serving variant)."""
+    water_side=1000 gives the 2M-triangle serving variant).  textured=False
+    gives the vase a plain grey diffuse colour instead of its texture (same
+    geometry)."""
     rng = np.random.default_rng(seed)
     meshes, mats = room_parts(rng)
-    mats += [Material("water", ks=(0.05, 0.05, 0.05), ni=1.33, d=0.0),
-             Material("vase", kd=(1.0, 1.0, 1.0), map_kd=f"noise_s{seed}.ppm"),
+    vase = (Material("vase", kd=(1.0, 1.0, 1.0), map_kd=f"noise_s{seed}.ppm") if textured
+            else Material("vase", kd=(0.6, 0.6, 0.6)))
+    mats += [Material("water", ks=(0.05, 0.05, 0.05), ni=1.33, d=0.0), vase,
              Material("mirror", ks=(0.95, 0.95, 0.95), ns=1000.0)]
     meshes.append(water_grid(rng, water_side, 0.8, "water", "water"))
     meshes.append(revolution(rng, (7.5, 0.0, 7.5), rev[0], rev[1], "vase", "vase"))
     meshes.append(uv_sphere((2.5, 5.0, 3.0), 1.2, sphere[0], sphere[1], "mirror", "mirror_ball"))
     os.makedirs(directory, exist_ok=True)
     write_ppm(os.path.join(directory, f"noise_s{seed}.ppm"), value_noise(rng))
-    name = f"full_s{seed}_w{water_side}.obj"
+    name = f"full_s{seed}_w{water_side}{'' if textured else '_plain'}.obj"
     return write_obj(os.path.join(directory, name), meshes, mats)
 
 

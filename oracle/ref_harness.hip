@@ -14,6 +14,7 @@
 // protocol for the list sizes.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -153,7 +154,9 @@ int ref_kdtree_intersect_timed(const char *co_path, const float *points, int64_t
 // texcoords 2, mesh and lights int4, materials 96 bytes, nodes 48 bytes.
 // seeds: W*ssx*H*ssy values; results: 3 floats per camera ray; s0_sizes
 // (may be NULL): sample*depth ints, the S0 count each level's intersect
-// ran over.
+// ran over; loop_seconds (may be NULL): host wall time from set_viewdirs to
+// the end of the last level (the part of run() after the constructor's
+// uploads, without get_image).
 int ref_render(const char *co_path, const float *points, int64_t n_points, const float *normals,
                int64_t n_normals, const float *texcoords, int64_t n_texcoords,
                const int32_t *mesh, int64_t n_tris, const void *materials, int64_t n_materials,
@@ -162,7 +165,7 @@ int ref_render(const char *co_path, const float *points, int64_t n_points, const
                const float *right, float width, float height, int32_t sample_x,
                int32_t sample_y, int32_t ssx, int32_t ssy, const float *background,
                int32_t sample, int32_t depth, int32_t disable_diffuse, const int64_t *seeds,
-               float *results, int32_t *s0_sizes) {
+               float *results, int32_t *s0_sizes, double *loop_seconds) {
     std::vector<void *> keep;
     hipModule_t mod = nullptr;
     int rc = -1;
@@ -243,6 +246,8 @@ int ref_render(const char *co_path, const float *points, int64_t n_points, const
         }
         if (!all) break;
 
+        if (!ok(hipDeviceSynchronize(), "uploads")) break;
+        const auto t_loop = std::chrono::steady_clock::now();
         // set_viewdirs[_vr] over exactly (gx, gy) items: the kernels test
         // `> sample_x`, so a larger range would write past the records.
         int32_t bx = 256;
@@ -346,6 +351,9 @@ int ref_render(const char *co_path, const float *points, int64_t n_points, const
             }
         }
         if (!fine) break;
+        if (loop_seconds)
+            *loop_seconds =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
         if (!ok(hipMemcpy(results, d_results, 12 * (size_t)R, hipMemcpyDeviceToHost),
                 "download results"))
             break;

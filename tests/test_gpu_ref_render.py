@@ -38,9 +38,10 @@ needs_ref = pytest.mark.skipif(not (os.path.exists(CO) and os.path.exists(LIB)),
                                reason="reference kernels not built (oracle/build_ref.sh)")
 
 
-def ref_render(kd, view, spp, depth, seed):
+def ref_render(kd, view, spp, depth, seed, timed=False):
     """results_mem of the reference's kernels for `view` (make_view's
-    arguments), and the S0 count each level's intersect ran over."""
+    arguments), the S0 count each level's intersect ran over, and with
+    `timed` the host wall time of the render loop in seconds."""
     # torch's HIP runtime must be the first in the process (it reports no
     # GPU after another copy has opened the device); loading the library
     # through _native does that before this harness's runtime starts.
@@ -50,7 +51,7 @@ def ref_render(kd, view, spp, depth, seed):
     i64, i32, f32 = C.c_int64, C.c_int32, C.c_float
     lib.ref_render.argtypes = [C.c_char_p, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64,
                                vp, i64, vp, i64, vp, vp, vp, f32, f32, i32, i32, i32, i32, vp,
-                               i32, i32, i32, vp, vp, vp]
+                               i32, i32, i32, vp, vp, vp, vp]
     lib.ref_last_error.restype = C.c_char_p
     sx, sy = view["sample_x"], view["sample_y"]
     ssx, ssy = view.get("supersample_x", 1), view.get("supersample_y", 1)
@@ -58,6 +59,7 @@ def ref_render(kd, view, spp, depth, seed):
     seeds = np.random.default_rng(seed).integers(0, 2 ** 31 - 1, R).astype(np.int64)
     acc = np.zeros((R, 3), np.float32)
     s0 = np.zeros((spp, depth), np.int32)
+    secs = C.c_double(0.0)
     f3 = lambda v: np.ascontiguousarray(v, np.float32)  # noqa: E731
     eye, up, right = f3(view["view_p"]), f3(view["top_dir"]), f3(view["right_dir"])
     bg = f3(view.get("background_color", (0.0, 0.0, 0.0)))
@@ -67,9 +69,9 @@ def ref_render(kd, view, spp, depth, seed):
                         *arr(kd.nodes), *arr(kd.leaf_data), eye.ctypes.data, up.ctypes.data,
                         right.ctypes.data, view["width"], view["height"], sx, sy, ssx, ssy,
                         bg.ctypes.data, spp, depth, view.get("disable_diffuse", 0),
-                        seeds.ctypes.data, acc.ctypes.data, s0.ctypes.data)
+                        seeds.ctypes.data, acc.ctypes.data, s0.ctypes.data, C.byref(secs))
     assert rc == 0, lib.ref_last_error()
-    return acc, s0
+    return (acc, s0, secs.value) if timed else (acc, s0)
 
 
 def gpu_render(kd, view, spp, depth, seed):
@@ -130,3 +132,41 @@ def test_reference_kernels_equirectangular_supersampled(room):
     print(f"\n[ref-render] equirectangular 2x2, {spp} spp: GPU vs reference kernels {frac:.4f} "
           f"within 4 SE (RMSE {rmse:.5f})")
     assert frac >= 0.99
+
+
+@needs_ref
+def test_reference_kernels_speed_1080p(tmp_path_factory):
+    """The 1080p workload (721 316 triangles, 1920x1080 with 2x2
+    supersampling, depth 8) with the vase untextured, 8 spp: the reference's
+    kernels under TraceRunner::run's host loop against this library's render
+    on the same GPU, rays counted as the reference's intersect launches count
+    them (primaries once, then every level's S0).  Prints both rates; the
+    live paths per level agree within 1 %."""
+    import time
+    from hptracing_amd.runner import TraceRunner, make_view
+    from hptracing_amd.scene import KDTree
+    import tempfile
+    path = scenes.cached(os.path.join(tempfile.gettempdir(), "hpt_scenes"), scenes.write_full, 0,
+                         water_side=500, textured=False)
+    kd = KDTree(path, os.path.dirname(path) + "/")
+    view = dict(**scenes.ROOM_VIEW, width=1920 / 1080, height=1.0, sample_x=1920, sample_y=1080,
+                supersample_x=2, supersample_y=2)
+    spp, depth = 8, 8
+    _, s0, ref_s = ref_render(kd, view, spp, depth, seed=31, timed=True)
+    R = 1920 * 1080 * 4
+    ref_rays = R + int(s0.sum())
+    r = TraceRunner(kd)
+    r.render(make_view(**view, sample=1, depth=depth, seed=5))  # warm
+    t0 = time.perf_counter()
+    r.render(make_view(**view, sample=spp, depth=depth, seed=32))
+    ours_s = time.perf_counter() - t0
+    ours_rays = int(r.last_stats.rays_cast)
+    live = np.array(r.last_stats.live_per_depth[:depth], np.int64)
+    ref_live = s0.sum(axis=0).astype(np.int64)
+    print(f"\n[ref-render] 1080p x4, depth 8, {spp} spp, untextured vase: reference kernels "
+          f"{ref_rays} rays in {ref_s:.3f}s = {ref_rays / ref_s / 1e6:.1f} Mrays/s; this library "
+          f"{ours_rays} rays in {ours_s:.3f}s = {ours_rays / ours_s / 1e6:.1f} Mrays/s "
+          f"({(ours_rays / ours_s) / (ref_rays / ref_s):.1f}x); live per level reference "
+          f"{ref_live[1:].tolist()} ours {live[1:].tolist()}")
+    rel = np.abs(live[1:] - ref_live[1:]) / ref_live[1:]
+    assert (rel < 0.01).all(), rel
