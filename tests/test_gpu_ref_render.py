@@ -74,9 +74,10 @@ def ref_render(kd, view, spp, depth, seed, timed=False):
     return (acc, s0, secs.value) if timed else (acc, s0)
 
 
-def gpu_render(kd, view, spp, depth, seed):
+def gpu_render(kd, view, spp, depth, seed, rocm_opencl=False):
     from hptracing_amd.runner import TraceRunner, make_view
     r = TraceRunner(kd)
+    r.set_arithmetic(rocm_opencl)
     _, acc = r.render(make_view(**view, sample=spp, depth=depth, seed=seed), return_accum=True)
     return acc, np.array(r.last_stats.live_per_depth[:depth], np.int64)
 
@@ -89,7 +90,17 @@ def test_reference_kernels_image_matches_gpu(room):
     """64x64 room view, depth 5, 1024 spp each: the reference kernels' image
     against this library's (standard errors from the per-pixel sample
     deviations of the float64 16384-spp render), both against that render,
-    and the live paths per bounce level."""
+    and the live paths per bounce level.
+
+    The live counts depend on the last bit of t: a light-branch path that
+    starts on the emitter samples a point in the emitter's own plane and is
+    kept when the direction's side matches (kernel.cl:527-548), which for a
+    hit point exactly in that plane is decided by rounding.  With ROCm's
+    OpenCL dot/cross in the cast (HPT_ARITH_ROCM_OPENCL, the arithmetic of
+    the reference build) every t, and so every such decision, is the
+    reference's, and the counts agree within 0.1 %; with the literal forms
+    fewer emitter hit points lie off the plane and 0.3 % more paths
+    continue."""
     ref64 = np.load(REF64)
     spp, depth = 1024, int(ref64["depth"])
     acc_ref, s0 = ref_render(room, ROOM64, spp, depth, seed=11)
@@ -101,16 +112,21 @@ def test_reference_kernels_image_matches_gpu(room):
     frac_r64, rmse_r64 = within_4se(mean_ref, spp, ref64)
     frac_g64, rmse_g64 = within_4se(mean_gpu, spp, ref64)
     ref_live = s0.sum(axis=0).astype(np.int64)
+    acc_ocl, live_ocl = gpu_render(room, ROOM64, spp, depth, seed=13, rocm_opencl=True)
+    frac_ocl, _ = frac_within_4se(acc_ocl.astype(np.float64) / spp, mean_ref, se)
     print(f"\n[ref-render] {spp} spp: GPU vs reference kernels {frac:.4f} within 4 SE "
-          f"(RMSE {rmse:.5f}); against float64: reference kernels {frac_r64:.4f} "
-          f"(RMSE {rmse_r64:.5f}), GPU {frac_g64:.4f} (RMSE {rmse_g64:.5f}); live paths per "
-          f"level reference {ref_live[1:].tolist()} GPU {live[1:].tolist()}")
-    assert frac >= 0.99
+          f"(RMSE {rmse:.5f}; with ROCm OpenCL arithmetic {frac_ocl:.4f}); against float64: "
+          f"reference kernels {frac_r64:.4f} (RMSE {rmse_r64:.5f}), GPU {frac_g64:.4f} "
+          f"(RMSE {rmse_g64:.5f}); live paths per level reference {ref_live[1:].tolist()} "
+          f"GPU {live[1:].tolist()}, with ROCm OpenCL arithmetic {live_ocl[1:].tolist()}")
+    assert frac >= 0.99 and frac_ocl >= 0.99
     # Against float64 both float32 renderers miss on the same pixels (camera
     # rays meeting a wall-corner edge pick the other wall in float32).
     assert abs(frac_r64 - frac_g64) < 0.005
+    rel = np.abs(live_ocl[1:] - ref_live[1:]) / ref_live[1:]
+    assert (rel < 0.001).all(), rel
     rel = np.abs(live[1:] - ref_live[1:]) / ref_live[1:]
-    assert (rel < 0.01).all(), rel
+    assert (rel < 0.005).all(), rel
 
 
 @needs_ref
