@@ -74,6 +74,15 @@ def ref_render(kd, view, spp, depth, seed, timed=False):
     return (acc, s0, secs.value) if timed else (acc, s0)
 
 
+def live_within_4_sigma(live, ref_live, n_paths):
+    """Per bounce level, whether two independent counts of surviving paths
+    out of n_paths differ by at most 4 standard deviations of the
+    difference (binomial, p = the reference's fraction)."""
+    p = ref_live / n_paths
+    sd = np.sqrt(2.0 * n_paths * p * (1.0 - p))
+    return np.abs(live - ref_live) <= 4.0 * sd + 1.0
+
+
 def gpu_render(kd, view, spp, depth, seed, rocm_opencl=False):
     from hptracing_amd.runner import TraceRunner, make_view
     r = TraceRunner(kd)
@@ -98,9 +107,9 @@ def test_reference_kernels_image_matches_gpu(room):
     hit point exactly in that plane is decided by rounding.  With ROCm's
     OpenCL dot/cross in the cast (HPT_ARITH_ROCM_OPENCL, the arithmetic of
     the reference build) every t, and so every such decision, is the
-    reference's, and the counts agree within 0.1 %; with the literal forms
-    fewer emitter hit points lie off the plane and 0.3 % more paths
-    continue."""
+    reference's, and the counts agree within 4 binomial standard deviations;
+    with the literal forms fewer emitter hit points lie off the plane and
+    0.3 % more paths continue."""
     ref64 = np.load(REF64)
     spp, depth = 1024, int(ref64["depth"])
     acc_ref, s0 = ref_render(room, ROOM64, spp, depth, seed=11)
@@ -123,8 +132,8 @@ def test_reference_kernels_image_matches_gpu(room):
     # Against float64 both float32 renderers miss on the same pixels (camera
     # rays meeting a wall-corner edge pick the other wall in float32).
     assert abs(frac_r64 - frac_g64) < 0.005
-    rel = np.abs(live_ocl[1:] - ref_live[1:]) / ref_live[1:]
-    assert (rel < 0.001).all(), rel
+    n_paths = 64 * 64 * spp
+    assert live_within_4_sigma(live_ocl[1:], ref_live[1:], n_paths).all()
     rel = np.abs(live[1:] - ref_live[1:]) / ref_live[1:]
     assert (rel < 0.005).all(), rel
 
@@ -150,8 +159,52 @@ def test_reference_kernels_equirectangular_supersampled(room):
     assert frac >= 0.99
 
 
+@pytest.fixture(scope="module")
+def full_plain():
+    """The 1080p workload's scene (721 316 triangles) with the vase
+    untextured: the harness passes no texture."""
+    import tempfile
+    from hptracing_amd.scene import KDTree
+    path = scenes.cached(os.path.join(tempfile.gettempdir(), "hpt_scenes"), scenes.write_full, 0,
+                         water_side=500, textured=False)
+    return KDTree(path, os.path.dirname(path) + "/")
+
+
+FULL_VIEW = dict(**scenes.ROOM_VIEW, width=1920 / 1080, height=1.0)
+
+
 @needs_ref
-def test_reference_kernels_speed_1080p(tmp_path_factory):
+def test_reference_kernels_image_full_scene(full_plain):
+    """The 1080p view of the full scene (water refraction, the mirror
+    sphere's specular bounces, diffuse and light) at 128x72 with 2x2
+    supersampling, depth 8, 256 spp.  Standard errors come from the spread
+    of sixteen independent 16-spp GPU renders; specular chains that reach
+    the emitter make some pixels heavy-tailed, so the bar is the same test
+    between two independent GPU images: the reference kernels' image must
+    score as well (within 0.5 %).  Live paths per level within 4 binomial
+    standard deviations with ROCm's OpenCL arithmetic in the cast."""
+    view = dict(**FULL_VIEW, sample_x=128, sample_y=72, supersample_x=2, supersample_y=2)
+    depth, spp, k = 8, 256, 16
+    acc_ref, s0 = ref_render(full_plain, view, spp, depth, seed=41)
+    chunks = np.stack([gpu_render(full_plain, view, spp // k, depth, seed=200 + j)[0]
+                       for j in range(k)]).astype(np.float64) / (spp // k)
+    mean_gpu = chunks.mean(axis=0)
+    se = np.sqrt(2.0) * chunks.std(axis=0, ddof=1) / np.sqrt(k)
+    frac, rmse = frac_within_4se(mean_gpu, acc_ref.astype(np.float64) / spp, se)
+    acc_b, _ = gpu_render(full_plain, view, spp, depth, seed=299)
+    frac_null, rmse_null = frac_within_4se(mean_gpu, acc_b.astype(np.float64) / spp, se)
+    _, live_ocl = gpu_render(full_plain, view, spp, depth, seed=300, rocm_opencl=True)
+    ref_live = s0.sum(axis=0).astype(np.int64)
+    print(f"\n[ref-render] full scene 128x72 x4, depth 8, {spp} spp: GPU vs reference kernels "
+          f"{frac:.4f} within 4 SE (RMSE {rmse:.5f}); GPU vs an independent GPU image "
+          f"{frac_null:.4f} (RMSE {rmse_null:.5f}); live per level reference "
+          f"{ref_live[1:].tolist()} GPU {live_ocl[1:].tolist()}")
+    assert frac >= frac_null - 0.005
+    assert live_within_4_sigma(live_ocl[1:], ref_live[1:], 128 * 72 * 4 * spp).all()
+
+
+@needs_ref
+def test_reference_kernels_speed_1080p(full_plain):
     """The 1080p workload (721 316 triangles, 1920x1080 with 2x2
     supersampling, depth 8) with the vase untextured, 8 spp: the reference's
     kernels under TraceRunner::run's host loop against this library's render
@@ -160,13 +213,8 @@ def test_reference_kernels_speed_1080p(tmp_path_factory):
     live paths per level agree within 1 %."""
     import time
     from hptracing_amd.runner import TraceRunner, make_view
-    from hptracing_amd.scene import KDTree
-    import tempfile
-    path = scenes.cached(os.path.join(tempfile.gettempdir(), "hpt_scenes"), scenes.write_full, 0,
-                         water_side=500, textured=False)
-    kd = KDTree(path, os.path.dirname(path) + "/")
-    view = dict(**scenes.ROOM_VIEW, width=1920 / 1080, height=1.0, sample_x=1920, sample_y=1080,
-                supersample_x=2, supersample_y=2)
+    kd = full_plain
+    view = dict(**FULL_VIEW, sample_x=1920, sample_y=1080, supersample_x=2, supersample_y=2)
     spp, depth = 8, 8
     _, s0, ref_s = ref_render(kd, view, spp, depth, seed=31, timed=True)
     R = 1920 * 1080 * 4
