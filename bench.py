@@ -289,12 +289,15 @@ def bench_serving(args):
     views = serving.request_views(args.serve_requests, seed=3, depth=args.depth)
     runner.render(views[0])  # warm
     # The launch forms are tuned on this scene's requests (the library's
-    # defaults are tuned on the 1080p scene): hpt_autotune on a 1280x720
-    # view, then on a panorama, of the mix.  Both the served and the
+    # defaults are tuned on the 1080p scene): hpt_autotune on the largest
+    # flat view of the mix (a 1280x720 one), then on a panorama.  Both the served and the
     # one-at-a-time runs below use the result.
     t1 = time.time()
-    tuned = runner.autotune(next(v for v in views if v.height > 0 and v.sample_x >= 1280))
+    flat = [v for v in views if v.height > 0]
     pano = [v for v in views if v.height < 0]
+    tuned = runner.get_tuning()
+    if flat:
+        tuned = runner.autotune(max(flat, key=lambda v: v.sample_x * v.sample_y))
     if pano:
         tuned = runner.autotune(pano[0])
     log(f"[bench] serving scene autotuned in {time.time() - t1:.1f}s: {tuned}")
