@@ -156,7 +156,9 @@ int ref_kdtree_intersect_timed(const char *co_path, const float *points, int64_t
 // (may be NULL): sample*depth ints, the S0 count each level's intersect
 // ran over; loop_seconds (may be NULL): host wall time from set_viewdirs to
 // the end of the last level (the part of run() after the constructor's
-// uploads, without get_image).
+// uploads, without get_image); primary_geometry (may be NULL): 4 ints per
+// camera ray, the int4 of the triangle its primary intersect found
+// (scene_mesh[geo_id], kernel.cl:187-192), or -1s for a miss.
 int ref_render(const char *co_path, const float *points, int64_t n_points, const float *normals,
                int64_t n_normals, const float *texcoords, int64_t n_texcoords,
                const int32_t *mesh, int64_t n_tris, const void *materials, int64_t n_materials,
@@ -165,7 +167,8 @@ int ref_render(const char *co_path, const float *points, int64_t n_points, const
                const float *right, float width, float height, int32_t sample_x,
                int32_t sample_y, int32_t ssx, int32_t ssy, const float *background,
                int32_t sample, int32_t depth, int32_t disable_diffuse, const int64_t *seeds,
-               float *results, int32_t *s0_sizes, double *loop_seconds) {
+               float *results, int32_t *s0_sizes, double *loop_seconds,
+               int32_t *primary_geometry) {
     std::vector<void *> keep;
     hipModule_t mod = nullptr;
     int rc = -1;
@@ -301,6 +304,23 @@ int ref_render(const char *co_path, const float *points, int64_t n_points, const
         if (!ok(hipMemcpy(d_cache, d_stage[1], 4 * (size_t)R, hipMemcpyDeviceToDevice), "cache s1"))
             break;
         const int32_t initial_s1 = sizes[1];
+        if (primary_geometry) {
+            std::vector<Unit> recs(R);
+            std::vector<int32_t> hits(initial_s1 > 0 ? initial_s1 : 1);
+            if (!ok(hipMemcpy(recs.data(), d_initial, sizeof(Unit) * (size_t)R,
+                              hipMemcpyDeviceToHost),
+                    "read records") ||
+                !ok(hipMemcpy(hits.data(), d_cache, 4 * (size_t)initial_s1,
+                              hipMemcpyDeviceToHost),
+                    "read s1"))
+                break;
+            for (size_t k = 0; k < 4 * (size_t)R; ++k) primary_geometry[k] = -1;
+            for (int32_t h = 0; h < initial_s1; ++h) {
+                const int32_t ray = hits[h];
+                if (ray < 0 || ray >= R) continue;
+                std::memcpy(primary_geometry + 4 * (size_t)ray, recs[ray].geometry, 16);
+            }
+        }
 
         bool fine = true;
         for (int32_t ii = 0; ii < sample && fine; ++ii) {

@@ -38,10 +38,11 @@ needs_ref = pytest.mark.skipif(not (os.path.exists(CO) and os.path.exists(LIB)),
                                reason="reference kernels not built (oracle/build_ref.sh)")
 
 
-def ref_render(kd, view, spp, depth, seed, timed=False):
+def ref_render(kd, view, spp, depth, seed, timed=False, primary=False):
     """results_mem of the reference's kernels for `view` (make_view's
-    arguments), the S0 count each level's intersect ran over, and with
-    `timed` the host wall time of the render loop in seconds."""
+    arguments), the S0 count each level's intersect ran over, with `timed`
+    the host wall time of the render loop in seconds, and with `primary`
+    the int4 of each camera ray's primary triangle (-1s: a miss)."""
     # torch's HIP runtime must be the first in the process (it reports no
     # GPU after another copy has opened the device); loading the library
     # through _native does that before this harness's runtime starts.
@@ -51,7 +52,7 @@ def ref_render(kd, view, spp, depth, seed, timed=False):
     i64, i32, f32 = C.c_int64, C.c_int32, C.c_float
     lib.ref_render.argtypes = [C.c_char_p, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64,
                                vp, i64, vp, i64, vp, vp, vp, f32, f32, i32, i32, i32, i32, vp,
-                               i32, i32, i32, vp, vp, vp, vp]
+                               i32, i32, i32, vp, vp, vp, vp, vp]
     lib.ref_last_error.restype = C.c_char_p
     sx, sy = view["sample_x"], view["sample_y"]
     ssx, ssy = view.get("supersample_x", 1), view.get("supersample_y", 1)
@@ -60,6 +61,7 @@ def ref_render(kd, view, spp, depth, seed, timed=False):
     acc = np.zeros((R, 3), np.float32)
     s0 = np.zeros((spp, depth), np.int32)
     secs = C.c_double(0.0)
+    geo = np.zeros((R, 4), np.int32)
     f3 = lambda v: np.ascontiguousarray(v, np.float32)  # noqa: E731
     eye, up, right = f3(view["view_p"]), f3(view["top_dir"]), f3(view["right_dir"])
     bg = f3(view.get("background_color", (0.0, 0.0, 0.0)))
@@ -69,9 +71,38 @@ def ref_render(kd, view, spp, depth, seed, timed=False):
                         *arr(kd.nodes), *arr(kd.leaf_data), eye.ctypes.data, up.ctypes.data,
                         right.ctypes.data, view["width"], view["height"], sx, sy, ssx, ssy,
                         bg.ctypes.data, spp, depth, view.get("disable_diffuse", 0),
-                        seeds.ctypes.data, acc.ctypes.data, s0.ctypes.data, C.byref(secs))
+                        seeds.ctypes.data, acc.ctypes.data, s0.ctypes.data, C.byref(secs),
+                        geo.ctypes.data if primary else None)
     assert rc == 0, lib.ref_last_error()
-    return (acc, s0, secs.value) if timed else (acc, s0)
+    out = (acc, s0) + ((secs.value,) if timed else ()) + ((geo,) if primary else ())
+    return out
+
+
+def primary_geometry(kd, view):
+    """This library's primary triangle (int4, -1s: a miss) per camera ray of
+    a pinhole view: k_camera's float32 arithmetic restated in numpy (each
+    operation rounded once, no contraction), cast by hpt_cast."""
+    import torch
+    from hptracing_amd.runner import TraceRunner
+    f = np.float32
+    sx = view["sample_x"] * view.get("supersample_x", 1)
+    sy = view["sample_y"] * view.get("supersample_y", 1)
+    up, right = np.array(view["top_dir"], f), np.array(view["right_dir"], f)
+    w, h = f(view["width"]), f(view["height"])
+    nd0 = np.array([up[1] * right[2] - up[2] * right[1], up[2] * right[0] - up[0] * right[2],
+                    up[0] * right[1] - up[1] * right[0]], f)
+    gy, gx = np.meshgrid(np.arange(sy), np.arange(sx), indexing="ij")
+    ty = gy.astype(f) / f(sy) * h - h * f(0.5)
+    tx = gx.astype(f) / f(sx) * w - w * f(0.5)
+    nd = nd0 + ty[..., None] * up
+    nd = (nd + tx[..., None] * right).reshape(-1, 3).astype(f)
+    length = np.sqrt((nd[:, 0] * nd[:, 0] + nd[:, 1] * nd[:, 1]) + nd[:, 2] * nd[:, 2])
+    d = (nd / length[:, None]).astype(f)
+    o = np.tile(np.array(view["view_p"], f), (len(d), 1))
+    ids, _ = TraceRunner(kd).cast(o, d)
+    torch.cuda.synchronize()
+    ids = ids.cpu().numpy()
+    return np.where(ids[:, None] >= 0, kd.geometries[np.maximum(ids, 0)], -1)
 
 
 def live_within_4_sigma(live, ref_live, n_paths):
@@ -157,6 +188,57 @@ def test_reference_kernels_equirectangular_supersampled(room):
     print(f"\n[ref-render] equirectangular 2x2, {spp} spp: GPU vs reference kernels {frac:.4f} "
           f"within 4 SE (RMSE {rmse:.5f})")
     assert frac >= 0.99
+
+
+@needs_ref
+def test_reference_kernels_no_diffuse_background(room):
+    """--no-diffuse (disable_diffuse: the diffuse share goes to light
+    sampling, kernel.cl:396-407) with a coloured background, 512 spp.
+    Standard errors come from sixteen independent GPU renders; light-sampled
+    paths that reach the emitter make pixels heavy-tailed, so the bar is the
+    same test between two independent GPU images (the reference kernels'
+    image within 0.5 % of it).  Direct light sampling makes pixel means
+    nearly noise-free, so the pixels whose camera ray meets a different
+    triangle in the two renderers -- rays on a triangle edge, where the
+    reference's OpenCL `normalize` and dot/cross and this library's literal
+    float32 forms round differently (wall corners pick the other wall, quad
+    diagonals crack) -- stand out; those whose material or hit differs are
+    counted (under 2 %) and left out.  Most paths here sample the light, and those that land on the
+    emitter sample again in its own plane, where the side test is decided by
+    rounding, so past the first level the live paths agree within 4
+    binomial standard deviations plus 1 % (the first level within 4
+    standard deviations)."""
+    view = dict(ROOM64, background_color=(0.2, 0.1, 0.05), disable_diffuse=1)
+    depth, spp, k = 5, 512, 16
+    acc_ref, s0, geo_ref = ref_render(room, view, spp, depth, seed=51, primary=True)
+    geo_ours = primary_geometry(room, view)
+    same_tri = (geo_ref == geo_ours).all(axis=1)
+    # a different triangle of the same material and plane (a quad's other
+    # half) shades the same; a different material or a miss does not
+    same = geo_ref[:, 3] == geo_ours[:, 3]
+    chunks = np.stack([gpu_render(room, view, spp // k, depth, seed=400 + j)[0]
+                       for j in range(k)]).astype(np.float64) / (spp // k)
+    mean_gpu = chunks.mean(axis=0)
+    se = np.sqrt(2.0) * chunks.std(axis=0, ddof=1) / np.sqrt(k)
+    frac, rmse = frac_within_4se(mean_gpu[same], acc_ref[same].astype(np.float64) / spp, se[same])
+    acc_b, _ = gpu_render(room, view, spp, depth, seed=499)
+    frac_null, rmse_null = frac_within_4se(mean_gpu[same], acc_b[same].astype(np.float64) / spp,
+                                           se[same])
+    _, live_ocl = gpu_render(room, view, spp, depth, seed=500, rocm_opencl=True)
+    ref_live = s0.sum(axis=0).astype(np.int64)
+    print(f"\n[ref-render] no-diffuse, background, {spp} spp: primary triangle differs on "
+          f"{int((~same_tri).sum())} of {len(same)} camera rays, its material (or hit) on "
+          f"{int((~same).sum())}; on the rest GPU vs reference kernels "
+          f"{frac:.4f} within 4 SE (RMSE {rmse:.5f}), GPU vs an independent GPU image "
+          f"{frac_null:.4f} (RMSE {rmse_null:.5f}); live per level reference "
+          f"{ref_live[1:].tolist()} GPU {live_ocl[1:].tolist()}")
+    assert (~same).mean() < 0.02
+    assert frac >= frac_null - 0.005
+    n_paths = 64 * 64 * spp
+    assert live_within_4_sigma(live_ocl[1:2], ref_live[1:2], n_paths).all()
+    p = ref_live[2:] / n_paths
+    allowed = 4.0 * np.sqrt(2.0 * n_paths * p * (1.0 - p)) + 0.01 * ref_live[2:]
+    assert (np.abs(live_ocl[2:] - ref_live[2:]) <= allowed).all()
 
 
 @pytest.fixture(scope="module")
