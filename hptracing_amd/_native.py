@@ -131,7 +131,7 @@ class Tuning(C.Structure):
     _fields_ = [("fast_division", C.c_int32), ("cast_blocks_per_cu", C.c_int32),
                 ("shade_blocks_per_cu", C.c_int32), ("traversal", C.c_int32),
                 ("refill_threshold", C.c_int32), ("node_steps", C.c_int32),
-                ("fetch", C.c_int32), ("walk_min", C.c_int32)]
+                ("fetch", C.c_int32), ("walk_min", C.c_int32), ("sort_rays", C.c_int32)]
 
 
 class ServerOpts(C.Structure):

@@ -158,6 +158,10 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
 
         std::unique_ptr<hpt_scene> s(new hpt_scene());
         hip_check(hipGetDevice(&s->device), "hipGetDevice");
+        for (int k = 0; k < 3; ++k) {  // the root's box orders sorted casts
+            s->box_lo[k] = nodes[0].box_start[k];
+            s->box_hi[k] = nodes[0].box_end[k];
+        }
         int64_t total = 0;
         upload(s->nodes, pnodes.data(), pnodes.size(), total);
         upload(s->tris, recs.data(), recs.size(), total);
@@ -269,6 +273,7 @@ int hpt_scene_get_tuning(const hpt_scene *s, hpt_tuning *out) {
         out->node_steps = s->dev.node_steps;
         out->fetch = s->dev.fetch;
         out->walk_min = s->dev.walk_min;
+        out->sort_rays = s->sort_rays;
     });
 }
 
@@ -279,7 +284,8 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
             t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64 ||
             t->traversal < 0 || t->traversal > 2 || t->refill_threshold < 1 ||
             t->refill_threshold > 64 || t->node_steps < 1 || t->node_steps > 4096 ||
-            !hpt::valid_fetch(t->fetch) || t->walk_min < 0 || t->walk_min > 64)
+            !hpt::valid_fetch(t->fetch) || t->walk_min < 0 || t->walk_min > 64 ||
+            t->sort_rays < 0 || t->sort_rays > 1)
             fail(HPT_ERR_INVALID, "tuning value out of range");
         std::lock_guard<std::mutex> lock(s->mu);
         s->dev.fast_div = t->fast_division ? 1 : 0;
@@ -288,6 +294,7 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
         s->dev.node_steps = t->node_steps;
         s->dev.fetch = t->fetch;
         s->dev.walk_min = t->walk_min;
+        s->sort_rays = t->sort_rays;
         s->cast_grid = t->cast_blocks_per_cu * s->n_cus;
         s->shade_grid = t->shade_blocks_per_cu * s->n_cus;
     });
@@ -315,6 +322,10 @@ static int cast_impl(hpt_scene *s, const float *org, const float *dir, int64_t n
         // NULL is HIP's null stream, so a cast follows whatever the caller
         // enqueued there to produce its rays.
         hipStream_t st = (hipStream_t)stream;
+        if (!counts && s->sort_rays && n >= hpt::kSortMinRays) {
+            hpt::cast_sorted(s, org, dir, n, tri_id, t, st);
+            return;
+        }
         hpt::launch_cast(s->dev, org, dir, nullptr, nullptr, hpt::CountRef{}, n, s->counter(),
                          tri_id, t, counts, nullptr, s->cast_grid, st);
         hip_check(hipGetLastError(), "k_cast launch");

@@ -180,6 +180,8 @@ struct hpt_scene {
     int64_t device_bytes = 0;
     int64_t n_nodes = 0, n_entries = 0, n_triangles = 0, n_textures = 0;
     int cast_grid = 0, shade_grid = 0, n_cus = 1;
+    int sort_rays = 0;                 // hpt_tuning.sort_rays
+    float box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // root node's box
     std::atomic<int> next_counter{0};
     std::mutex mu;
     hpt::Workspace ws;
@@ -191,3 +193,12 @@ struct hpt_scene {
     }
 };
 
+namespace hpt {
+// Smallest batch that sort_rays orders: below it the sort costs more than
+// the locality gains.
+constexpr int64_t kSortMinRays = 65536;
+// hpt_cast of a batch in Morton order of its origins (sort.hip); results
+// in the caller's order.
+void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
+                 float *out_t, hipStream_t st);
+}  // namespace hpt

@@ -302,6 +302,7 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
         base.node_steps = s->dev.node_steps;
         base.fetch = s->dev.fetch;
         base.walk_min = s->dev.walk_min;
+        base.sort_rays = s->sort_rays;
         base.cast_blocks_per_cu = s->cast_grid / s->n_cus;
         base.shade_blocks_per_cu = s->shade_grid / s->n_cus;
     }
@@ -313,6 +314,7 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
         s->dev.node_steps = t.node_steps;
         s->dev.fetch = t.fetch;
         s->dev.walk_min = t.walk_min;
+        s->sort_rays = t.sort_rays;
         s->cast_grid = std::max(1, t.cast_blocks_per_cu) * s->n_cus;
         s->shade_grid = std::max(1, t.shade_blocks_per_cu) * s->n_cus;
     };
@@ -351,6 +353,7 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
         }
         try_knob([](hpt_tuning &t, int v) { t.fast_division = v; }, {0, 1});
         try_knob([](hpt_tuning &t, int v) { t.cast_blocks_per_cu = v; }, {3, 4, 5, 6, 7, 8});
+        if (!shade) try_knob([](hpt_tuning &t, int v) { t.sort_rays = v; }, {0, 1});
     }
     if (shade) {
         const int socc = std::max(1, base.shade_blocks_per_cu);
@@ -443,8 +446,13 @@ int hpt_autotune_cast(hpt_scene *s, const float *org, const float *dir, int64_t 
             for (int rep = 0; rep < 2; ++rep) {
                 KernelTimer timer(true, st);
                 timer.begin();
-                launch_cast(s->dev, org, dir, nullptr, nullptr, CountRef{}, m, s->counter(),
-                            ids.as<int>(), ts.as<float>(), nullptr, nullptr, s->cast_grid, st);
+                // as hpt_cast runs it, the sort of sort_rays included
+                if (s->sort_rays && m >= kSortMinRays)
+                    cast_sorted(s, org, dir, m, ids.as<int>(), ts.as<float>(), st);
+                else
+                    launch_cast(s->dev, org, dir, nullptr, nullptr, CountRef{}, m, s->counter(),
+                                ids.as<int>(), ts.as<float>(), nullptr, nullptr, s->cast_grid,
+                                st);
                 timer.end();
                 hip_check(hipStreamSynchronize(st), "autotune cast");
                 best = std::min(best, timer.total_ms());

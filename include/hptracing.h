@@ -147,8 +147,8 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
  * hpt_scene_create (fast_division = 1, traversal = 2, refill_threshold = 48,
- * node_steps = 24, fetch = 157, walk_min = 0, cast grid = min(6, occupancy)
- * x CUs, shade grid = occupancy x CUs). */
+ * node_steps = 24, fetch = 157, walk_min = 0, sort_rays = 0, cast grid =
+ * min(6, occupancy) x CUs, shade grid = occupancy x CUs). */
 typedef struct {
     int32_t fast_division;       /* slab quotients via certified reciprocals */
     int32_t cast_blocks_per_cu;  /* persistent grid of the cast kernel      */
@@ -180,6 +180,11 @@ typedef struct {
     int32_t walk_min;            /* traversal 2: a round's node phase ends
                                     once fewer lanes than this (0..64) can
                                     walk (0: only when none can)            */
+    int32_t sort_rays;           /* hpt_cast: order a batch of at least
+                                    65536 rays by the Morton code of its
+                                    origins in the scene box before casting
+                                    (results still in the caller's order;
+                                    pays for incoherent batches)            */
 } hpt_tuning;
 int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);

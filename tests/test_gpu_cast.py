@@ -175,3 +175,32 @@ def test_autotune_cast_keeps_results(soup_small):
     torch.cuda.synchronize()
     assert torch.equal(ids0, ids1)
     assert torch.equal(t0.view(torch.int32), t1.view(torch.int32))
+
+
+@pytest.mark.parametrize("fetch", [157, 220, 0])
+def test_cast_sorted_batches(room, soup_small, oracle_mod, fetch):
+    """sort_rays casts a batch of at least 65536 rays in the Morton order of
+    its origins and writes the results back in the caller's order: bit for
+    bit the unsorted cast's, just below, at and above that size, with
+    origins outside the scene box and zero directions included, and a
+    subset equals the oracle's."""
+    import torch
+    from hptracing_amd.runner import TraceRunner
+    for kd, lo, hi, seed in ((soup_small, (0, 0, 0), (1, 1, 1), 61),
+                             (room, (-5, -5, -8), (15, 15, 15), 62)):
+        r = TraceRunner(kd)
+        r.set_tuning(fetch=fetch)
+        for n in (65535, 65536, 150001):
+            org, d = scenes.random_rays(n, lo, hi, seed=seed + n)
+            d[:64] = 0.0
+            r.set_tuning(sort_rays=0)
+            ids0, t0 = r.cast(org, d)
+            r.set_tuning(sort_rays=1)
+            ids1, t1 = r.cast(org, d)
+            torch.cuda.synchronize()
+            assert torch.equal(ids0, ids1), (fetch, n)
+            assert torch.equal(t0.view(torch.int32), t1.view(torch.int32)), (fetch, n)
+        sub = np.random.default_rng(seed).choice(n, 10000, replace=False)
+        ref_ids, ref_t = oracle_mod.Scene(kd).kd_cast(org[sub], d[sub])
+        assert np.array_equal(ids1.cpu().numpy()[sub], ref_ids)
+        assert np.array_equal(t1.cpu().numpy()[sub].view(np.int32), ref_t.view(np.int32))
