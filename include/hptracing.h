@@ -184,7 +184,10 @@ typedef struct {
                                     65536 rays by the Morton code of its
                                     origins in the scene box before casting
                                     (results still in the caller's order;
-                                    pays for incoherent batches)            */
+                                    pays for incoherent batches).  Uses
+                                    stream-ordered allocations and a radix
+                                    sort on the caller's stream: keep it off
+                                    for casts captured in a hipGraph      */
 } hpt_tuning;
 int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);
