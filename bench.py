@@ -78,7 +78,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(the sharding design is described in DESIGN.md)"}))
+                          "(sample-space or tile sharding across GPUs is a design, not code)"}))
         sys.exit(2)
 
     import numpy as np
