@@ -204,3 +204,20 @@ def test_cast_sorted_batches(room, soup_small, oracle_mod, fetch):
         ref_ids, ref_t = oracle_mod.Scene(kd).kd_cast(org[sub], d[sub])
         assert np.array_equal(ids1.cpu().numpy()[sub], ref_ids)
         assert np.array_equal(t1.cpu().numpy()[sub].view(np.int32), ref_t.view(np.int32))
+
+
+def test_tuning_rejects_out_of_range(room):
+    """hpt_scene_set_tuning refuses values outside each knob's range (and
+    forms that are not compiled) and leaves the scene's tuning unchanged."""
+    from hptracing_amd._native import HptError
+    from hptracing_amd.runner import TraceRunner
+    r = TraceRunner(room)
+    before = r.get_tuning()
+    for bad in (dict(sort_rays=2), dict(sort_rays=-1), dict(fetch=6), dict(traversal=3),
+                dict(refill_threshold=0), dict(refill_threshold=65), dict(node_steps=0),
+                dict(walk_min=65), dict(cast_blocks_per_cu=0), dict(shade_blocks_per_cu=65)):
+        with pytest.raises(HptError):
+            r.set_tuning(**bad)
+        assert r.get_tuning() == before, bad
+    r.set_tuning(sort_rays=1)
+    assert r.get_tuning()["sort_rays"] == 1
