@@ -111,7 +111,9 @@ int usage(const char *argv0) {
                  "usage: %s -i scene.obj -o out.png [--mtl dir/] [--width 500] [--height 500]\n"
                  "  [--view 0,0,-500] [--up 0,1,0] [--right 1,0,0] [--angle 1.0 | -1 for VR]\n"
                  "  [-x] [-s 10] [-d 6] [--brightness 1] [--no-diffuse] [--background 0,0,0]\n"
-                 "  [--seed N] [--progressive PREFIX]\n"
+                 "  [--seed N] [--progressive PREFIX] [--tune]\n"
+                 "  --tune runs hpt_autotune on the view before rendering (launch forms\n"
+                 "  only; the image is the same)\n"
                  "  --progressive writes PREFIX<passes>.png each time the render server\n"
                  "  publishes more sample passes (the refining frames the original tool's\n"
                  "  window shows); -o is then optional\n", argv0);
@@ -125,7 +127,7 @@ int main(int argc, char **argv) {
                                                  right = "1,0,0", background = "0,0,0";
     int width = 500, height = 500, sample = 10, depth = 6;
     float angle = 1.0f, brightness = 1.0f;
-    bool supersample = false, no_diffuse = false;
+    bool supersample = false, no_diffuse = false, tune = false;
     unsigned long long seed = 1;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -154,6 +156,7 @@ int main(int argc, char **argv) {
         else if (a == "--seed") { next(v); seed = std::strtoull(v.c_str(), nullptr, 10); }
         else if (a == "-x" || a == "--supersample") supersample = true;
         else if (a == "--no-diffuse") no_diffuse = true;
+        else if (a == "--tune") tune = true;
         else if (a == "-h" || a == "--help") return usage(argv[0]);
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
@@ -200,6 +203,16 @@ int main(int argc, char **argv) {
     v.disable_diffuse = no_diffuse ? 1 : 0;
     v.brightness = brightness;
     v.seed = seed;
+    if (tune) {
+        hpt_tuning chosen;
+        if (hpt_autotune(scene, &v, 0, &chosen) != HPT_OK) {
+            std::fprintf(stderr, "autotune: %s\n", hpt_last_error());
+            return 1;
+        }
+        std::fprintf(stderr, "Tuned: fetch %d, %d cast blocks per CU, refill %d, node steps %d\n",
+                     chosen.fetch, chosen.cast_blocks_per_cu, chosen.refill_threshold,
+                     chosen.node_steps);
+    }
     std::vector<uint8_t> rgba((size_t)width * height * 4);
     // hptracing.cpp:285-295: the image is written flipped in y.
     auto save = [&](const std::string &path) {

@@ -95,3 +95,19 @@ def test_cli_progressive_frames(tmp_path, room):
         img = read_png(f"{prefix}{s}.png")
         assert np.array_equal(img, r.render(v)[::-1, :, :3]), s
     assert np.array_equal(read_png(out), read_png(f"{prefix}20.png"))
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+def test_cli_tune_same_image(tmp_path):
+    """--tune changes launch forms only: the file is the untuned one's, byte
+    for byte."""
+    obj = scenes.write_room(str(tmp_path / "scene"), 0)
+    base = [CLI, "-i", obj, "--width", "40", "--height", "24", "-s", "3", "-d", "4",
+            "--view", "5,5,-6", "--right", "-1,0,0", "--seed", "9"]
+    a, b = str(tmp_path / "a.ppm"), str(tmp_path / "b.ppm")
+    res = subprocess.run(base + ["-o", a], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    res = subprocess.run(base + ["-o", b, "--tune"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "Tuned:" in res.stderr
+    assert open(a, "rb").read() == open(b, "rb").read()
