@@ -88,6 +88,14 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         // triangle records in leaf order.
         const int64_t nn = d->n_nodes;
         const hpt_kd_node *nodes = d->nodes;
+        // escape_indices follows parent links and reads the node they name:
+        // every link must name a node (or be -1) before it runs.
+        for (int64_t i = 0; i < nn; ++i) {
+            const hpt_kd_node &nd = nodes[i];
+            if (nd.child < -1 || nd.child >= nn || nd.parent < -1 || nd.parent >= nn ||
+                nd.sibling < -1 || nd.sibling >= nn)
+                fail(HPT_ERR_SCENE, "KD node link out of range");
+        }
         std::vector<int32_t> esc = escape_indices(nodes, nn);
         std::vector<float> pnodes(8 * nn);
         std::vector<float> recs;
@@ -191,8 +199,6 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         upload(s->materials, d->materials, (size_t)nm, total);
         upload(s->lights, d->lights, 4 * (size_t)d->n_lights, total);
         upload(s->textures, d->textures, (size_t)d->n_textures * HPT_TEXTURE_SIZE * HPT_TEXTURE_SIZE * 4, total);
-        std::vector<hpt::WorkCounter> zero(kCounterRing, hpt::WorkCounter{0, 0});
-        upload(s->counters, zero.data(), zero.size(), total);
         s->device_bytes = total;
         s->n_nodes = nn;
         s->n_entries = entries;
@@ -219,7 +225,11 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.fast_div = 1;
         ds.boxes_fast = boxes_fast ? 1 : 0;
         ds.traversal = 2;
-        ds.arith_ocl = 0;
+        // dot/cross as the reference build evaluates them: on this GPU the
+        // reference's kernel runs through ROCm's OpenCL device library,
+        // which fuses them (HPT_ARITH_ROCM_OPENCL); its hit distances are
+        // then reproduced bit for bit.
+        ds.arith_ocl = 1;
         ds.refill = 48;
         ds.node_steps = 24;
         ds.fetch = 157;
@@ -326,8 +336,9 @@ static int cast_impl(hpt_scene *s, const float *org, const float *dir, int64_t n
             hpt::cast_sorted(s, org, dir, n, tri_id, t, st);
             return;
         }
-        hpt::launch_cast(s->dev, org, dir, nullptr, nullptr, hpt::CountRef{}, n, s->counter(),
-                         tri_id, t, counts, nullptr, s->cast_grid, st);
+        hpt::StreamCounter wc(st);
+        hpt::launch_cast(s->dev, org, dir, nullptr, nullptr, hpt::CountRef{}, n, wc.p, tri_id, t,
+                         counts, nullptr, s->cast_grid, st);
         hip_check(hipGetLastError(), "k_cast launch");
     });
 }

@@ -93,6 +93,11 @@ struct Workspace {
     DevMem prim_org, prim_dir, prim_thr, prim_id, prim_t, prim_hit, prim_list;
     DevMem a_org, a_dir, a_thr, a_seed, b_org, b_dir, b_thr, b_seed;
     DevMem hit_id, hit_t, accum, rgba, views, segs, live, prim_count, totals, pass, passes;
+    // The work counter of every cast this workspace's owner launches.  All
+    // of them run on the owner's one stream, one after another, and each
+    // launch's last wave re-arms the counter, so one counter serves them
+    // all and no other caller can reach it.
+    DevMem counter;
     int64_t path_capacity = 0;
     // rays: camera rays (and accumulator entries per slice); paths: live
     // paths of one launch (up to kPassSlices per camera ray).
@@ -124,6 +129,12 @@ struct Workspace {
             hit_t.reserve(4 * n);
             path_capacity = paths;
         }
+        if (!counter.p) {
+            const WorkCounter zero{0, 0};
+            counter.reserve(sizeof(WorkCounter));
+            hip_check(hipMemcpy(counter.p, &zero, sizeof(zero), hipMemcpyHostToDevice),
+                      "work counter");
+        }
         rgba.reserve(4 * (size_t)std::max<int64_t>(pixels, 1));
         views.reserve(sizeof(hpt::ViewParams) * (size_t)std::max(n_views, 1));
         segs.reserve(sizeof(hpt::Segment) * (size_t)std::max(n_passes * n_views, 1));
@@ -135,7 +146,31 @@ struct Workspace {
     }
 };
 
-constexpr int kCounterRing = 256;
+// The work counter of one hpt_cast call, from the stream-ordered allocator
+// and released on the same stream: casts running concurrently on other
+// streams (or threads) never share a counter.  Cleared by a fill kernel, so
+// a captured graph holds no memset node.
+struct StreamCounter {
+    hipStream_t st;
+    WorkCounter *p = nullptr;
+    explicit StreamCounter(hipStream_t s) : st(s) {
+        void *q = nullptr;
+        hip_check(hipMallocAsync(&q, sizeof(WorkCounter), st), "hipMallocAsync");
+        p = static_cast<WorkCounter *>(q);
+        launch_zero_counter(p, st);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(p, st);
+            p = nullptr;
+            hip_check(e, "k_zero_counter");
+        }
+    }
+    StreamCounter(const StreamCounter &) = delete;
+    StreamCounter &operator=(const StreamCounter &) = delete;
+    ~StreamCounter() {
+        if (p) (void)hipFreeAsync(p, st);
+    }
+};
 
 // Device time of a series of launches on one stream, from HIP event pairs
 // recorded around each launch (read after the stream is synchronized).
@@ -176,21 +211,15 @@ struct hpt_scene {
     int device = 0;
     hpt::DevScene dev{};
     hpt::DevMem nodes, tris, tri_recs, leaf_ids, points, normals, texcoords, mesh, materials,
-        lights, textures, counters;
+        lights, textures;
     int64_t device_bytes = 0;
     int64_t n_nodes = 0, n_entries = 0, n_triangles = 0, n_textures = 0;
     int cast_grid = 0, shade_grid = 0, n_cus = 1;
     int sort_rays = 0;                 // hpt_tuning.sort_rays
     float box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // root node's box
-    std::atomic<int> next_counter{0};
     std::mutex mu;
-    hpt::Workspace ws;
+    hpt::Workspace ws;  // hpt_render / hpt_batch_render / hpt_autotune, under mu
     hipStream_t stream = nullptr;
-
-    hpt::WorkCounter *counter() {
-        const int k = next_counter.fetch_add(1) % hpt::kCounterRing;
-        return counters.as<hpt::WorkCounter>() + k;
-    }
 };
 
 namespace hpt {

@@ -1237,6 +1237,15 @@ __global__ void k_next_pass(int *pass) { *pass += 1; }
 
 void launch_next_pass(int *pass, hipStream_t st) { k_next_pass<<<1, 1, 0, st>>>(pass); }
 
+__global__ void k_zero_counter(WorkCounter *wc) {
+    wc->next = 0;
+    wc->done = 0;
+}
+
+void launch_zero_counter(WorkCounter *wc, hipStream_t st) {
+    k_zero_counter<<<1, 1, 0, st>>>(wc);
+}
+
 void launch_resolve(const ViewParams *views, int view_id, int width, int height, int ssx, int ssy,
                     int samples, float brightness, const float *accum, uchar4 *rgba,
                     hipStream_t st) {

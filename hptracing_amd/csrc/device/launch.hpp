@@ -28,6 +28,7 @@ void launch_shade(const DevScene &sc, const ViewParams *views, int depth_index, 
                   Paths in, const int *hit_id, const float *hit_t, Paths out, CountRef n_out,
                   float *accum, int grid, hipStream_t st);
 void launch_next_pass(int *pass, hipStream_t st);
+void launch_zero_counter(WorkCounter *wc, hipStream_t st);
 void launch_resolve(const ViewParams *views, int view_id, int width, int height, int ssx, int ssy,
                     int samples, float brightness, const float *accum, uchar4 *rgba,
                     hipStream_t st);

@@ -91,8 +91,8 @@ struct PassLauncher {
             if (i > 0) {
                 if (cast_timer) cast_timer->begin();
                 launch_cast(s->dev, nullptr, nullptr, cur.org, cur.dir, CountRef{live, pass, stride, i},
-                            0, s->counter(), ws.hit_id.as<int>(), ws.hit_t.as<float>(), nullptr,
-                            totals, s->cast_grid, st);
+                            0, ws.counter.as<WorkCounter>(), ws.hit_id.as<int>(),
+                            ws.hit_t.as<float>(), nullptr, totals, s->cast_grid, st);
                 if (cast_timer) cast_timer->end();
             }
             if (shade_timer) shade_timer->begin();
@@ -171,8 +171,9 @@ void render_batch(hpt_scene *s, const hpt_view *views, int n, uint8_t *const *rg
     Paths prim{ws.prim_org.as<float4>(), ws.prim_dir.as<float4>(), ws.prim_thr.as<float4>(), nullptr};
     for (int k = 0; k < n; ++k) launch_camera(d_views, k, jobs[k].R, prim, st);
     cast_timer.begin();
-    launch_cast(s->dev, nullptr, nullptr, prim.org, prim.dir, CountRef{}, rays, s->counter(),
-                ws.prim_id.as<int>(), ws.prim_t.as<float>(), nullptr, totals, s->cast_grid, st);
+    launch_cast(s->dev, nullptr, nullptr, prim.org, prim.dir, CountRef{}, rays,
+                ws.counter.as<WorkCounter>(), ws.prim_id.as<int>(), ws.prim_t.as<float>(), nullptr,
+                totals, s->cast_grid, st);
     cast_timer.end();
     for (int k = 0; k < n; ++k)
         launch_primary(d_views, k, jobs[k].R, ws.prim_id.as<int>(), ws.prim_t.as<float>(),
@@ -444,13 +445,14 @@ int hpt_autotune_cast(hpt_scene *s, const float *org, const float *dir, int64_t 
         auto measure = [&](bool) {
             double best = 1e30;
             for (int rep = 0; rep < 2; ++rep) {
+                StreamCounter wc(st);
                 KernelTimer timer(true, st);
                 timer.begin();
                 // as hpt_cast runs it, the sort of sort_rays included
                 if (s->sort_rays && m >= kSortMinRays)
                     cast_sorted(s, org, dir, m, ids.as<int>(), ts.as<float>(), st);
                 else
-                    launch_cast(s->dev, org, dir, nullptr, nullptr, CountRef{}, m, s->counter(),
+                    launch_cast(s->dev, org, dir, nullptr, nullptr, CountRef{}, m, wc.p,
                                 ids.as<int>(), ts.as<float>(), nullptr, nullptr, s->cast_grid,
                                 st);
                 timer.end();

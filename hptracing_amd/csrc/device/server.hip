@@ -194,7 +194,7 @@ void hpt_server::start_requests(const std::vector<Request *> &admitted) {
         hip_check(hipMemsetAsync(prim_count + r->slot, 0, sizeof(int), st), "memset");
         launch_camera(d_views, r->slot, r->R, prim, st);
         launch_cast(scene->dev, nullptr, nullptr, prim.org + r->base, prim.dir + r->base,
-                    CountRef{}, r->R, scene->counter(), ws.prim_id.as<int>() + r->base,
+                    CountRef{}, r->R, ws.counter.as<WorkCounter>(), ws.prim_id.as<int>() + r->base,
                     ws.prim_t.as<float>() + r->base, nullptr, nullptr, scene->cast_grid, st);
         launch_primary(d_views, r->slot, r->R, ws.prim_id.as<int>(), ws.prim_t.as<float>(),
                        ws.accum.as<float>(), ws.prim_hit.as<int2>(), ws.prim_list.as<int>(),
@@ -249,7 +249,7 @@ void hpt_server::one_pass(std::vector<Request *> &finished, std::vector<Request 
         for (int i = 0; i < depth; ++i) {
             if (i > 0)
                 launch_cast(scene->dev, nullptr, nullptr, cur.org, cur.dir,
-                            CountRef{live, pass, stride, i}, 0, scene->counter(),
+                            CountRef{live, pass, stride, i}, 0, ws.counter.as<WorkCounter>(),
                             ws.hit_id.as<int>(), ws.hit_t.as<float>(), nullptr, nullptr,
                             scene->cast_grid, st);
             launch_shade(scene->dev, d_views, i, CountRef{live, pass, stride, i}, cur,

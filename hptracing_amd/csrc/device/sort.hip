@@ -127,8 +127,9 @@ void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, 
               "radix sort");
     k_gather_rays<<<grid, kBlock, 0, st>>>(org3, dir3, idx_out, n, org4, dir4);
     hip_check(hipGetLastError(), "k_gather_rays");
-    launch_cast(s->dev, nullptr, nullptr, org4, dir4, CountRef{}, n, s->counter(), ids_s, t_s,
-                nullptr, nullptr, s->cast_grid, st);
+    StreamCounter wc(st);
+    launch_cast(s->dev, nullptr, nullptr, org4, dir4, CountRef{}, n, wc.p, ids_s, t_s, nullptr,
+                nullptr, s->cast_grid, st);
     hip_check(hipGetLastError(), "k_cast launch");
     k_scatter_hits<<<grid, kBlock, 0, st>>>(idx_out, ids_s, t_s, n, out_id, out_t);
     hip_check(hipGetLastError(), "k_scatter_hits");

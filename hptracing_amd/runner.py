@@ -86,8 +86,9 @@ class TraceRunner:
         return {k: getattr(t, k) for k, _ in N.Tuning._fields_}
 
     def set_arithmetic(self, rocm_opencl: bool) -> None:
-        """dot/cross of the intersection test: literal (default) or the fused
-        forms of ROCm's OpenCL builtins."""
+        """dot/cross of the intersection test: the fused forms of ROCm's
+        OpenCL builtins (default; the reference kernel's bits) or the literal
+        formulas with every product and sum rounded."""
         N.check_device(N.device().hpt_scene_set_arithmetic(self._h, 1 if rocm_opencl else 0))
 
     def run(self, view_p, top_dir, right_dir, width, height, sample_x, sample_y,

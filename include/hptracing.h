@@ -193,12 +193,15 @@ int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);
 
 /* Evaluation of OpenCL's dot() and cross() in the intersection test.
- * LITERAL (default): the textbook formulas with every product and sum
- * rounded, (a.x*b.x + a.y*b.y) + a.z*b.z and a.y*b.z - a.z*b.y ..., which
- * is what kernel.cl writes.  ROCM_OPENCL: the fused forms ROCm's OpenCL
- * device library uses for those builtins, so that hit distances equal the
- * reference kernel's bit for bit when it is compiled by that library.  Hit
- * triangles agree in both modes except on exact ties. */
+ * ROCM_OPENCL (default): the fused forms ROCm's OpenCL device library uses
+ * for those builtins, dot = fma(a.z, b.z, fma(a.y, b.y, a.x*b.x)) and one
+ * fma per cross component, so that hit triangles AND distances equal the
+ * reference kernel's bit for bit (kernel.cl:85-109 compiled by that
+ * library, which is where the reference's CL_DEVICE_TYPE_GPU host runs on
+ * this hardware).  LITERAL: the textbook formulas with every product and
+ * sum rounded, (a.x*b.x + a.y*b.y) + a.z*b.z and a.y*b.z - a.z*b.y ...;
+ * hit triangles agree with the default except on exact ties, distances
+ * within 1e-6 relative on 99.7 % (sliver soup) to 99.99 % (room) of hits. */
 #define HPT_ARITH_LITERAL 0
 #define HPT_ARITH_ROCM_OPENCL 1
 int hpt_scene_set_arithmetic(hpt_scene *scene, int mode);

@@ -68,7 +68,7 @@ int orc_box_intersect(const float bs[3], const float be[3], const float sp[3], c
  * and sum; ORC_ARITH_ROCM_OPENCL fuses them as ROCm's OpenCL device library
  * does (tests/golden/rocm_ocl_builtins/samples.npz holds the library's
  * outputs). */
-static int g_arith = ORC_ARITH_LITERAL;
+static int g_arith = ORC_ARITH_ROCM_OPENCL;
 void orc_set_arithmetic(int mode) { g_arith = mode; }
 
 static inline float dot_ocl(v3 a, v3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }

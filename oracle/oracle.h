@@ -69,9 +69,10 @@ typedef struct {
 #define ORC_MAX_MATCH_DATA 512
 
 /* Evaluation of dot/cross in the intersection test (process-wide setting;
- * set it before casting).  LITERAL: every product and sum rounded, as
- * kernel.cl writes them.  ROCM_OPENCL: the fused forms of ROCm's OpenCL
- * builtins.  orc_dot_cross evaluates both for a pin against the library's
+ * set it before casting).  ROCM_OPENCL (default, the library's default
+ * too): the fused forms of ROCm's OpenCL builtins, i.e. the reference
+ * kernel as that library compiles it.  LITERAL: every product and sum
+ * rounded, as kernel.cl writes them.  orc_dot_cross evaluates both for a pin against the library's
  * recorded outputs (4 floats per pair: dot, cross.xyz). */
 #define ORC_ARITH_LITERAL 0
 #define ORC_ARITH_ROCM_OPENCL 1

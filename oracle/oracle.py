@@ -227,7 +227,8 @@ ARITH_ROCM_OPENCL = 1
 
 
 def set_arithmetic(mode: int) -> None:
-    """dot/cross evaluation of the intersection test (process-wide)."""
+    """dot/cross evaluation of the intersection test (process-wide;
+    ARITH_ROCM_OPENCL by default, as in the library)."""
     lib().orc_set_arithmetic(mode)
 
 

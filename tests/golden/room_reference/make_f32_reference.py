@@ -1,11 +1,15 @@
 """Writes reference_f32.npz: the same room view as reference.npz rendered by
-the float32 oracle (oracle/oracle.c, the reference algorithm's arithmetic)
-at 16384 samples per pixel, with its RMSE against the float64 render.
+the float32 oracle (oracle/oracle.c, the reference algorithm's arithmetic;
+dot/cross of the intersection test fused as ROCm's OpenCL builtins fuse
+them, the oracle's and the library's default) at 16384 samples per pixel,
+with its RMSE against the float64 render.
 
 Float32 and float64 renders differ deterministically on pixels whose camera
 ray meets an edge shared by two surfaces (the room's corner lines): the two
-precisions assign the pixel to different surfaces, and since the primary hit
-is the same for every pass, no number of samples removes it.  So the per-pixel
+precisions assign the pixel to different surfaces (and with the fused forms
+a few camera rays slip through the crack of a quad's diagonal, as the
+reference kernel's do), and since the primary hit is the same for every
+pass, no number of samples removes it.  So the per-pixel
 4-standard-error test at equal spp is made against this float32 render, and
 the float64 render is the RMSE yardstick.  Run from the repository root:
 
@@ -40,12 +44,13 @@ def main():
                     sample_x=int(ref["width"]), sample_y=int(ref["height"]), sample=spp,
                     depth=int(ref["depth"]), seed=SEED)
     t0 = time.time()
+    O.set_arithmetic(O.ARITH_ROCM_OPENCL)
     acc, _, _ = O.Scene(kd).render(v)
     mean = acc.astype(np.float64) / spp
     rmse = float(np.sqrt(np.mean((mean - ref["mean"]) ** 2)))
     out = os.path.join(here, "reference_f32.npz")
     np.savez_compressed(out, mean=mean.astype(np.float32), spp=spp, seed=SEED,
-                        rmse_vs_f64=rmse)
+                        rmse_vs_f64=rmse, arithmetic=O.ARITH_ROCM_OPENCL)
     print(f"wrote {out} in {time.time() - t0:.1f}s; RMSE vs float64 {rmse:.5f}")
 
 

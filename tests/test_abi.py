@@ -51,3 +51,21 @@ def test_null_arguments_are_rejected_without_a_device():
     host = N.host()
     out = C.c_void_p()
     assert host.hpt_host_load_obj(None, None, 0, C.byref(out)) == N.HPT_ERR_INVALID
+
+
+@pytest.mark.parametrize("field,value", [("parent", 10**6), ("sibling", 10**6), ("child", -7),
+                                         ("parent", -2)])
+def test_scene_create_rejects_node_links_out_of_range(cornell, field, value):
+    """hpt_scene_create validates every KD link before it follows any: a
+    parent, sibling or child outside [-1, n_nodes) is a scene error, found
+    on the host before the first device call."""
+    import numpy as np
+    dev = N.device()
+    d = cornell.desc()
+    nodes = np.array(cornell.nodes, copy=True)
+    nodes[len(nodes) // 2][field] = value
+    d.nodes = nodes.ctypes.data
+    out = C.c_void_p()
+    assert dev.hpt_scene_create(C.byref(d), C.byref(out)) == N.HPT_ERR_SCENE
+    assert b"link out of range" in dev.hpt_last_error()
+    assert not out.value

@@ -9,7 +9,8 @@ memory, so 200k is the largest size it builds.
 The oracle cannot walk 4M rays in a test's time, so the full batch is
 checked through properties that do not depend on size: every reported t is
 reproduced bit for bit by Moller-Trumbore on the reported triangle (float32,
-the kernel's operation order, evaluated here in numpy), every miss and hit
+the kernel's operation order with dot/cross fused as in the default
+arithmetic, evaluated in numpy by tests/f32ops.py), every miss and hit
 flag is consistent, and a seeded random subset of the rays matches the
 oracle exactly (ids and t)."""
 from __future__ import annotations
@@ -17,33 +18,10 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from f32ops import moller_trumbore
 from hptracing_amd import scenes
 
 pytestmark = pytest.mark.gpu
-
-
-def mt_float32(org, d, pa, pb, pc):
-    f = np.float32
-    e1, e2 = (pb - pa).astype(f), (pc - pa).astype(f)
-
-    def cross(a, b):
-        return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1],
-                         a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
-                         a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], 1).astype(f)
-
-    def dot(a, b):
-        return ((a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]).astype(f)
-
-    P = cross(d, e2)
-    with np.errstate(all="ignore"):
-        inv = (f(1) / dot(e1, P)).astype(f)
-        T = (org - pa).astype(f)
-        u = (dot(T, P) * inv).astype(f)
-        Q = cross(T, e1)
-        v = (dot(d, Q) * inv).astype(f)
-        t = (dot(e2, Q) * inv).astype(f)
-    ok = ~((u < 0) | (u > 1) | (v < 0) | (u + v > 1)) & (t > 0)
-    return np.where(ok, t, f(-1)), u, v
 
 
 def test_cast_full_size_properties(soup_200k, oracle_mod):
@@ -61,7 +39,8 @@ def test_cast_full_size_properties(soup_200k, oracle_mod):
     assert 0.05 < hit.mean() < 0.999
     assert (t[~hit] == np.float32(-42)).all()
     tri = kd.geometries[ids[hit]]
-    t_re, _, _ = mt_float32(org[hit], d[hit], pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]])
+    t_re, _, _ = moller_trumbore(org[hit], d[hit], pts[tri[:, 0]], pts[tri[:, 1]],
+                                 pts[tri[:, 2]], fused=True)
     assert np.array_equal(t_re.view(np.int32), t[hit].view(np.int32))
     # a seeded subset against the oracle, bit for bit
     rng = np.random.default_rng(9)

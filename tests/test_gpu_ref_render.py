@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, frac_within_4se, within_4se
-from hptracing_amd import _native as N
+import ref_abi
 from hptracing_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -31,10 +31,9 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_DIR = os.path.join(ROOT, "oracle", "_ref")
 CO = os.path.join(REF_DIR, "ref_kernels_notex.co")
-LIB = os.path.join(REF_DIR, "libref_harness.so")
 REF64 = os.path.join(GOLDEN, "room_reference", "reference.npz")
 
-needs_ref = pytest.mark.skipif(not (os.path.exists(CO) and os.path.exists(LIB)),
+needs_ref = pytest.mark.skipif(not (os.path.exists(CO) and ref_abi.available()),
                                reason="reference kernels not built (oracle/build_ref.sh)")
 
 
@@ -43,17 +42,7 @@ def ref_render(kd, view, spp, depth, seed, timed=False, primary=False):
     arguments), the S0 count each level's intersect ran over, with `timed`
     the host wall time of the render loop in seconds, and with `primary`
     the int4 of each camera ray's primary triangle (-1s: a miss)."""
-    # torch's HIP runtime must be the first in the process (it reports no
-    # GPU after another copy has opened the device); loading the library
-    # through _native does that before this harness's runtime starts.
-    N.device()
-    lib = C.CDLL(LIB)
-    vp = C.c_void_p
-    i64, i32, f32 = C.c_int64, C.c_int32, C.c_float
-    lib.ref_render.argtypes = [C.c_char_p, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64,
-                               vp, i64, vp, i64, vp, vp, vp, f32, f32, i32, i32, i32, i32, vp,
-                               i32, i32, i32, vp, vp, vp, vp, vp]
-    lib.ref_last_error.restype = C.c_char_p
+    lib = ref_abi.load()
     sx, sy = view["sample_x"], view["sample_y"]
     ssx, ssy = view.get("supersample_x", 1), view.get("supersample_y", 1)
     R = sx * ssx * sy * ssy
@@ -114,10 +103,14 @@ def live_within_4_sigma(live, ref_live, n_paths):
     return np.abs(live - ref_live) <= 4.0 * sd + 1.0
 
 
-def gpu_render(kd, view, spp, depth, seed, rocm_opencl=False):
+def gpu_render(kd, view, spp, depth, seed, rocm_opencl=None):
+    """This library's accumulator and live paths per level, in the default
+    arithmetic (ROCm OpenCL dot/cross in the cast) unless `rocm_opencl` is
+    given."""
     from hptracing_amd.runner import TraceRunner, make_view
     r = TraceRunner(kd)
-    r.set_arithmetic(rocm_opencl)
+    if rocm_opencl is not None:
+        r.set_arithmetic(rocm_opencl)
     _, acc = r.render(make_view(**view, sample=spp, depth=depth, seed=seed), return_accum=True)
     return acc, np.array(r.last_stats.live_per_depth[:depth], np.int64)
 
@@ -138,9 +131,9 @@ def test_reference_kernels_image_matches_gpu(room):
     hit point exactly in that plane is decided by rounding.  With ROCm's
     OpenCL dot/cross in the cast (HPT_ARITH_ROCM_OPENCL, the arithmetic of
     the reference build) every t, and so every such decision, is the
-    reference's, and the counts agree within 4 binomial standard deviations;
-    with the literal forms fewer emitter hit points lie off the plane and
-    0.3 % more paths continue."""
+    reference's, and the counts of the default arithmetic agree within 4
+    binomial standard deviations; with the literal forms fewer emitter hit
+    points lie off the plane and 0.3 % more paths continue."""
     ref64 = np.load(REF64)
     spp, depth = 1024, int(ref64["depth"])
     acc_ref, s0 = ref_render(room, ROOM64, spp, depth, seed=11)
@@ -152,20 +145,20 @@ def test_reference_kernels_image_matches_gpu(room):
     frac_r64, rmse_r64 = within_4se(mean_ref, spp, ref64)
     frac_g64, rmse_g64 = within_4se(mean_gpu, spp, ref64)
     ref_live = s0.sum(axis=0).astype(np.int64)
-    acc_ocl, live_ocl = gpu_render(room, ROOM64, spp, depth, seed=13, rocm_opencl=True)
-    frac_ocl, _ = frac_within_4se(acc_ocl.astype(np.float64) / spp, mean_ref, se)
+    acc_lit, live_lit = gpu_render(room, ROOM64, spp, depth, seed=13, rocm_opencl=False)
+    frac_lit, _ = frac_within_4se(acc_lit.astype(np.float64) / spp, mean_ref, se)
     print(f"\n[ref-render] {spp} spp: GPU vs reference kernels {frac:.4f} within 4 SE "
-          f"(RMSE {rmse:.5f}; with ROCm OpenCL arithmetic {frac_ocl:.4f}); against float64: "
+          f"(RMSE {rmse:.5f}; with the literal arithmetic {frac_lit:.4f}); against float64: "
           f"reference kernels {frac_r64:.4f} (RMSE {rmse_r64:.5f}), GPU {frac_g64:.4f} "
           f"(RMSE {rmse_g64:.5f}); live paths per level reference {ref_live[1:].tolist()} "
-          f"GPU {live[1:].tolist()}, with ROCm OpenCL arithmetic {live_ocl[1:].tolist()}")
-    assert frac >= 0.99 and frac_ocl >= 0.99
+          f"GPU {live[1:].tolist()}, with the literal arithmetic {live_lit[1:].tolist()}")
+    assert frac >= 0.99 and frac_lit >= 0.99
     # Against float64 both float32 renderers miss on the same pixels (camera
     # rays meeting a wall-corner edge pick the other wall in float32).
     assert abs(frac_r64 - frac_g64) < 0.005
     n_paths = 64 * 64 * spp
-    assert live_within_4_sigma(live_ocl[1:], ref_live[1:], n_paths).all()
-    rel = np.abs(live[1:] - ref_live[1:]) / ref_live[1:]
+    assert live_within_4_sigma(live[1:], ref_live[1:], n_paths).all()
+    rel = np.abs(live_lit[1:] - ref_live[1:]) / ref_live[1:]
     assert (rel < 0.005).all(), rel
 
 
@@ -200,8 +193,8 @@ def test_reference_kernels_no_diffuse_background(room):
     image within 0.5 % of it).  Direct light sampling makes pixel means
     nearly noise-free, so the pixels whose camera ray meets a different
     triangle in the two renderers -- rays on a triangle edge, where the
-    reference's OpenCL `normalize` and dot/cross and this library's literal
-    float32 forms round differently (wall corners pick the other wall, quad
+    reference's OpenCL `normalize` in set_viewdirs and this library's
+    literal float32 camera round differently (wall corners pick the other wall, quad
     diagonals crack) -- stand out; those whose material or hit differs are
     counted (under 2 %) and left out.  Most paths here sample the light, and those that land on the
     emitter sample again in its own plane, where the side test is decided by
@@ -224,7 +217,7 @@ def test_reference_kernels_no_diffuse_background(room):
     acc_b, _ = gpu_render(room, view, spp, depth, seed=499)
     frac_null, rmse_null = frac_within_4se(mean_gpu[same], acc_b[same].astype(np.float64) / spp,
                                            se[same])
-    _, live_ocl = gpu_render(room, view, spp, depth, seed=500, rocm_opencl=True)
+    _, live_ocl = gpu_render(room, view, spp, depth, seed=500)
     ref_live = s0.sum(axis=0).astype(np.int64)
     print(f"\n[ref-render] no-diffuse, background, {spp} spp: primary triangle differs on "
           f"{int((~same_tri).sum())} of {len(same)} camera rays, its material (or hit) on "
@@ -264,7 +257,8 @@ def test_reference_kernels_image_full_scene(full_plain):
     the emitter make some pixels heavy-tailed, so the bar is the same test
     between two independent GPU images: the reference kernels' image must
     score as well (within 0.5 %).  Live paths per level within 4 binomial
-    standard deviations with ROCm's OpenCL arithmetic in the cast."""
+    standard deviations (the default arithmetic: ROCm's OpenCL dot/cross in
+    the cast)."""
     view = dict(**FULL_VIEW, sample_x=128, sample_y=72, supersample_x=2, supersample_y=2)
     depth, spp, k = 8, 256, 16
     acc_ref, s0 = ref_render(full_plain, view, spp, depth, seed=41)
@@ -275,7 +269,7 @@ def test_reference_kernels_image_full_scene(full_plain):
     frac, rmse = frac_within_4se(mean_gpu, acc_ref.astype(np.float64) / spp, se)
     acc_b, _ = gpu_render(full_plain, view, spp, depth, seed=299)
     frac_null, rmse_null = frac_within_4se(mean_gpu, acc_b.astype(np.float64) / spp, se)
-    _, live_ocl = gpu_render(full_plain, view, spp, depth, seed=300, rocm_opencl=True)
+    _, live_ocl = gpu_render(full_plain, view, spp, depth, seed=300)
     ref_live = s0.sum(axis=0).astype(np.int64)
     print(f"\n[ref-render] full scene 128x72 x4, depth 8, {spp} spp: GPU vs reference kernels "
           f"{frac:.4f} within 4 SE (RMSE {rmse:.5f}); GPU vs an independent GPU image "
