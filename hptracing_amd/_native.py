@@ -124,6 +124,7 @@ HPT_RENDER_COUNT_WORK = 2
 HPT_RENDER_NO_GRAPH = 4
 HPT_RENDER_PASS_AT_A_TIME = 8
 HPT_CAST_COUNTS = 5
+HPT_NOALLOC_MAX_RAYS = 1 << 20
 HPT_SERVE_PROGRESSIVE = 1
 
 
@@ -206,6 +207,7 @@ def device() -> C.CDLL:
         lib.hpt_scene_stats_get.argtypes = [vp, C.POINTER(SceneStats)]
         lib.hpt_cast.argtypes = [vp, vp, vp, i64, vp, vp, vp]
         lib.hpt_cast_counted.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp]
+        lib.hpt_cast_noalloc.argtypes = [vp, vp, vp, i64, vp, vp, vp]
         lib.hpt_render.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(RenderStats)]
         lib.hpt_render_ex.argtypes = [vp, C.POINTER(View), vp, vp, C.POINTER(RenderStats),
                                       C.c_uint]

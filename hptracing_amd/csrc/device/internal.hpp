@@ -227,7 +227,23 @@ namespace hpt {
 // the locality gains.
 constexpr int64_t kSortMinRays = 65536;
 // hpt_cast of a batch in Morton order of its origins (sort.hip); results
-// in the caller's order.
+// in the caller's order.  Buffers from the stream-ordered allocator.
 void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
                  float *out_t, hipStream_t st);
+// The same with the caller's buffers, each of n entries (keys and idx
+// twice: the radix sort's double buffer), temp_bytes >= sort_temp_bytes(n)
+// and a work counter cleared to zero.
+struct SortBuffers {
+    uint32_t *keys[2];
+    int32_t *idx[2];
+    float4 *org4, *dir4;
+    int *ids;
+    float *t;
+    void *temp;
+    size_t temp_bytes;
+    WorkCounter *wc;
+};
+size_t sort_temp_bytes(int64_t n, hipStream_t st);
+void cast_sorted_into(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
+                      float *out_t, const SortBuffers &b, hipStream_t st);
 }  // namespace hpt

@@ -16,8 +16,9 @@
 //   (the cast, path form)
 //   k_scatter_hits  hit id and t back to the caller's positions
 //
-// The buffers come from the stream-ordered allocator and are released on
-// the same stream, so concurrent casts on other streams share nothing.
+// hpt_cast takes the buffers from the stream-ordered allocator and releases
+// them on the same stream, so concurrent casts on other streams share
+// nothing; hpt_cast_noalloc carves them out of its static scratch array.
 #include <hipcub/hipcub.hpp>
 
 #include "internal.hpp"
@@ -95,18 +96,17 @@ struct AsyncBuffers {
 
 }  // namespace
 
-void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
-                 float *out_t, hipStream_t st) {
-    AsyncBuffers buf(st);
-    uint32_t *keys_in = buf.get<uint32_t>((size_t)n);
-    uint32_t *keys_out = buf.get<uint32_t>((size_t)n);
-    int32_t *idx_in = buf.get<int32_t>((size_t)n);
-    int32_t *idx_out = buf.get<int32_t>((size_t)n);
-    float4 *org4 = buf.get<float4>((size_t)n);
-    float4 *dir4 = buf.get<float4>((size_t)n);
-    int *ids_s = buf.get<int>((size_t)n);
-    float *t_s = buf.get<float>((size_t)n);
+size_t sort_temp_bytes(int64_t n, hipStream_t st) {
+    size_t bytes = 0;
+    hipcub::DoubleBuffer<uint32_t> keys(nullptr, nullptr);
+    hipcub::DoubleBuffer<int32_t> idx(nullptr, nullptr);
+    hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys, idx, (int)n, 0, 30, st),
+              "radix sort size");
+    return bytes;
+}
 
+void cast_sorted_into(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
+                      float *out_t, const SortBuffers &b, hipStream_t st) {
     float3 lo, scale;
     float *lo_p = &lo.x, *scale_p = &scale.x;
     for (int k = 0; k < 3; ++k) {
@@ -115,24 +115,43 @@ void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, 
         scale_p[k] = ext > 0.f ? 1023.f / ext : 0.f;
     }
     const int grid = std::max(1, s->n_cus) * 8;
-    k_ray_keys<<<grid, kBlock, 0, st>>>(org3, n, lo, scale, keys_in, idx_in);
+    k_ray_keys<<<grid, kBlock, 0, st>>>(org3, n, lo, scale, b.keys[0], b.idx[0]);
     hip_check(hipGetLastError(), "k_ray_keys");
-    size_t temp_bytes = 0;
-    hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys_in, keys_out, idx_in,
-                                                 idx_out, (int)n, 0, 30, st),
-              "radix sort size");
-    void *temp = buf.get<uint8_t>(temp_bytes);
-    hip_check(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, idx_in,
-                                                 idx_out, (int)n, 0, 30, st),
+    // The double-buffer form of the sort needs temporary storage for its
+    // digit counts only; which buffer holds the result is known on the
+    // host when the call returns.
+    hipcub::DoubleBuffer<uint32_t> keys(b.keys[0], b.keys[1]);
+    hipcub::DoubleBuffer<int32_t> idx(b.idx[0], b.idx[1]);
+    size_t temp_bytes = b.temp_bytes;
+    hip_check(hipcub::DeviceRadixSort::SortPairs(b.temp, temp_bytes, keys, idx, (int)n, 0, 30, st),
               "radix sort");
-    k_gather_rays<<<grid, kBlock, 0, st>>>(org3, dir3, idx_out, n, org4, dir4);
+    const int32_t *order = idx.Current();
+    k_gather_rays<<<grid, kBlock, 0, st>>>(org3, dir3, order, n, b.org4, b.dir4);
     hip_check(hipGetLastError(), "k_gather_rays");
-    StreamCounter wc(st);
-    launch_cast(s->dev, nullptr, nullptr, org4, dir4, CountRef{}, n, wc.p, ids_s, t_s, nullptr,
+    launch_cast(s->dev, nullptr, nullptr, b.org4, b.dir4, CountRef{}, n, b.wc, b.ids, b.t, nullptr,
                 nullptr, s->cast_grid, st);
     hip_check(hipGetLastError(), "k_cast launch");
-    k_scatter_hits<<<grid, kBlock, 0, st>>>(idx_out, ids_s, t_s, n, out_id, out_t);
+    k_scatter_hits<<<grid, kBlock, 0, st>>>(order, b.ids, b.t, n, out_id, out_t);
     hip_check(hipGetLastError(), "k_scatter_hits");
+}
+
+void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
+                 float *out_t, hipStream_t st) {
+    AsyncBuffers buf(st);
+    SortBuffers b{};
+    for (int k = 0; k < 2; ++k) {
+        b.keys[k] = buf.get<uint32_t>((size_t)n);
+        b.idx[k] = buf.get<int32_t>((size_t)n);
+    }
+    b.org4 = buf.get<float4>((size_t)n);
+    b.dir4 = buf.get<float4>((size_t)n);
+    b.ids = buf.get<int>((size_t)n);
+    b.t = buf.get<float>((size_t)n);
+    b.temp_bytes = sort_temp_bytes(n, st);
+    b.temp = buf.get<uint8_t>(b.temp_bytes);
+    StreamCounter wc(st);
+    b.wc = wc.p;
+    cast_sorted_into(s, org3, dir3, n, out_id, out_t, b, st);
 }
 
 }  // namespace hpt

@@ -186,8 +186,8 @@ typedef struct {
                                     (results still in the caller's order;
                                     pays for incoherent batches).  Uses
                                     stream-ordered allocations and a radix
-                                    sort on the caller's stream: keep it off
-                                    for casts captured in a hipGraph      */
+                                    sort on the caller's stream
+                                    (hpt_cast_noalloc: a static array)    */
 } hpt_tuning;
 int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);
@@ -213,7 +213,10 @@ int hpt_scene_set_arithmetic(hpt_scene *scene, int mode);
  * reference keeps only the triangle's int4 (kernel.cl:191); the index is the
  * position of that int4 in the mesh.  n <= 2^30 per call.  Asynchronous on
  * `stream` (a hipStream_t; NULL is the null stream, ordered with the work
- * of every blocking stream). */
+ * of every blocking stream).  Each call takes its work counter from the
+ * stream-ordered allocator, so concurrent calls on other streams or threads
+ * share nothing; callers that cannot allocate (graph capture) use
+ * hpt_cast_noalloc. */
 int hpt_cast(hpt_scene *scene, const float *org, const float *dir, int64_t n,
              int32_t *tri_id, float *t, void *stream);
 
@@ -226,6 +229,20 @@ int hpt_cast(hpt_scene *scene, const float *org, const float *dir, int64_t n,
 int hpt_cast_counted(hpt_scene *scene, const float *org, const float *dir,
                      int64_t n, int32_t *tri_id, float *t, int32_t *counts,
                      void *stream);
+
+/* hpt_cast without any device allocation: same arguments, same results bit
+ * for bit.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  hpt_cast takes a stream-ordered
+ * work counter per call (and sort buffers when the scene's sort_rays is
+ * on); this entry point uses a static device array of 64 MiB + 256 bytes
+ * instead, which each GPU holds from the library's load until the process
+ * ends.  n <= HPT_NOALLOC_MAX_RAYS (larger n: HPT_ERR_INVALID, nothing
+ * launched).  The current device must be the scene's.  Calls that overlap
+ * on one GPU share that array and race: serialise them (one stream). */
+#define HPT_NOALLOC_MAX_RAYS (1 << 20)
+int hpt_cast_noalloc(hpt_scene *scene, const float *org, const float *dir,
+                     int64_t n, int32_t *tri_id, float *t, void *stream);
 
 /* One render request: the argument list of `TraceRunner::run`
  * (trace_runner.h:53-65), plus an explicit RNG seed (the reference seeds
