@@ -224,26 +224,9 @@ def measured_traffic(args):
 def bench_batching(runner, n_views, depth):
     """64 uneven views of the same scene in one batch (shared launches, one
     captured pass graph) against the same views rendered one at a time."""
-    import math
+    from hptracing_amd import serving
 
-    import numpy as np
-
-    from hptracing_amd import scenes
-    from hptracing_amd.runner import make_view
-
-    rng = np.random.default_rng(7)
-    views = []
-    for i in range(n_views):
-        w = int(rng.integers(128, 1025))
-        h = int(rng.integers(128, 769))
-        spp = int(rng.integers(4, 65))
-        pos = rng.uniform([1.5, 1.5, 1.5], [scenes.ROOM - 1.5, scenes.ROOM - 2.0, scenes.ROOM - 1.5])
-        yaw = rng.uniform(0, 2 * math.pi)
-        pitch = rng.uniform(-0.3, 0.3)
-        up = (math.sin(pitch) * -math.sin(yaw), math.cos(pitch), math.sin(pitch) * math.cos(yaw))
-        right = (math.cos(yaw), 0.0, math.sin(yaw))
-        views.append(make_view(tuple(pos), up, right, w / h, 1.0, w, h, 1, 1, (0, 0, 0), spp,
-                               depth, 0, 1.0, seed=500 + i))
+    views = serving.batch_views(n_views, seed=7, depth=depth)
     runner.render_batch(views[:2])  # warm the path
     t0 = time.perf_counter()
     runner.render_batch(views)

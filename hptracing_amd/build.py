@@ -65,10 +65,23 @@ def build_cli() -> str:
     return out
 
 
+def build_boundary_client() -> str:
+    """tests/cpp/boundary_client: a test program written against the C++
+    boundary (include/hptracing.hpp) the way the reference's CLI uses its
+    classes."""
+    src = os.path.join(ROOT, "tests", "cpp", "boundary_client.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "boundary_client")
+    _run(["g++", *HOST_FLAGS, f"-I{INCLUDE}", src, "-o", out + ".tmp", f"-L{LIB_DIR}",
+          "-lhpt_host", "-lhptracing", "-Wl,-rpath,$ORIGIN/../../hptracing_amd/lib"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> None:
     build_host()
     build_device()
     build_cli()
+    build_boundary_client()
 
 
 if __name__ == "__main__":

@@ -53,6 +53,27 @@ def request_views(n: int, seed: int = 0, depth: int = 8, scale: float = 1.0) -> 
     return views
 
 
+def batch_views(n: int = 64, seed: int = 7, depth: int = 8) -> List[N.View]:
+    """The batching workload: n pinhole views of the full scene with
+    resolutions uniform in [128, 1024] x [128, 768], spp uniform in
+    [4, 64], positions uniform inside the room and view/up vectors of
+    random yaw and pitch (|pitch| <= 0.3 rad)."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for i in range(n):
+        w = int(rng.integers(128, 1025))
+        h = int(rng.integers(128, 769))
+        spp = int(rng.integers(4, 65))
+        pos = rng.uniform([1.5, 1.5, 1.5], [ROOM - 1.5, ROOM - 2.0, ROOM - 1.5])
+        yaw = rng.uniform(0, 2 * math.pi)
+        pitch = rng.uniform(-0.3, 0.3)
+        up = (math.sin(pitch) * -math.sin(yaw), math.cos(pitch), math.sin(pitch) * math.cos(yaw))
+        right = (math.cos(yaw), 0.0, math.sin(yaw))
+        views.append(make_view(tuple(pos), up, right, w / h, 1.0, w, h, 1, 1, (0, 0, 0), spp,
+                               depth, 0, 1.0, seed=500 + i))
+    return views
+
+
 def camera_samples(v: N.View) -> int:
     return v.sample_x * v.supersample_x * v.sample_y * v.supersample_y * v.sample
 
