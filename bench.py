@@ -10,30 +10,35 @@ like the reference's intersect launches: primaries once plus every
 continuation ray of every pass.
 
 Besides the timed render the script reports:
-  roofline      the cast kernel (dominant): algorithmic bytes -- the
-                information content of the reference's search: 32 B per ray
-                (origin and direction in, hit id and t out), 40 B per node
-                box-tested (box and links), 4 B per leaf collected (list
-                entry), 56 B per triangle test (list entry, int4, three
-                vertices) -- over its summed launch time from HIP events on
-                the render stream, against 8 TB/s HBM.  `traffic` is the HBM bytes per
-                launch that rocprofv3 counted (FETCH_SIZE, WRITE_SIZE,
-                scaled by a copy-kernel calibration) for this workload, read
-                from profiles/cast_traffic_full1080p.json when present.
+  roofline      the cast kernel (dominant): `achieved` is its algorithmic
+                bytes -- the information content of the reference's search:
+                32 B per ray (origin and direction in, hit id and t out),
+                40 B per node box-tested (box and links), 4 B per leaf
+                collected (list entry), 56 B per triangle test (list entry,
+                int4, three vertices) -- over its average launch time from
+                HIP events on the render stream, against 8 TB/s HBM.
+                `traffic` is the HBM bytes per launch rocprofv3 counted
+                (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known size) and
+                `issue` the VALU side of the same kernel (busy fraction,
+                active lanes, instructions per node visit and per triangle
+                test), both read from profiles/cast_counters_full1080p.json
+                when that file was measured on the same kernel sources,
+                tuning and workload (else null).
   cpu_baseline  the float32 C restatement of the reference kernels (oracle/)
-                on the host cores, on a bounded sample of the same camera
-                rays at the same spp.
-  cast_only     the ray-cast stage in isolation (hpt_cast), with the default
-                launch forms and after hpt_autotune_cast: 2^22 rays with
-                origins uniform in the scene box and directions uniform on
-                the sphere through a seeded triangle soup (centres uniform
-                in the unit cube, edges of log-uniform length in
-                [1e-3, 1e-1]) of 200 000 triangles, the largest the
-                reference's KD builder fits in memory; device time from HIP
-                events, and its algorithmic GB/s from the same batch's
-                counted walk.
-
+                on the host cores: camera rays sampled in 32 bands spread
+                evenly over the frame, at the same spp; one warm-up, then
+                the median of three timed runs.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Profiling helpers (not benchmark runs):
+  python bench.py --calibrate-copy
+        3 device copies of 1 GiB: what FETCH_SIZE / WRITE_SIZE read per byte
+        moved (run under rocprofv3 --pmc).
+  python bench.py --summarize-counters DIR
+        reads the rocprofv3 outputs of the counter runs under DIR (fetch/,
+        write/, sq/, l2/, cal_fetch/, cal_write/, trace/, and bench_config.json
+        written by --profile-record DIR) and writes
+        profiles/cast_counters_full1080p.json.
 """
 from __future__ import annotations
 
@@ -74,7 +79,20 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[2.0, 8.0])
     ap.add_argument("--cast-tris", type=int, default=200_000,
                     help="triangles of the cast-only soup (0 skips the cast-only leg)")
+    ap.add_argument("--profile-record", metavar="DIR",
+                    help="write DIR/bench_config.json (kernel sources, tuning, workload)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the copy of known size for the byte counters")
+    ap.add_argument("--summarize-counters", metavar="DIR",
+                    help="only write the counter record from the rocprofv3 runs under DIR")
     args = ap.parse_args()
+
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters)
+        return
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -129,14 +147,18 @@ def main():
     cst = runner.last_stats
     cast_bytes = (32 * int(cst.rays_cast) + 40 * int(cst.node_visits) + 4 * int(cst.leaf_visits)
                   + 56 * int(cst.tri_tests))
-    cast_s = tst.cast_ms / 1e3
-    achieved = cast_bytes / cast_s / 1e9 if cast_s > 0 else 0.0
     per_launch_ms = tst.cast_ms / max(1, tst.cast_launches)
     log(f"[bench] profile ({pk} passes): cast {tst.cast_ms:.1f} ms in {tst.cast_launches} "
         f"launches, shade {tst.shade_ms:.1f} ms in {tst.shade_launches}; "
         f"visits {cst.node_visits}, leaves {cst.leaf_visits}, tests {cst.tri_tests}")
-
-    traffic = measured_traffic(args)
+    config = bench_config(args, runner)
+    if args.profile_record:
+        os.makedirs(args.profile_record, exist_ok=True)
+        with open(os.path.join(args.profile_record, "bench_config.json"), "w") as f:
+            json.dump(config, f, indent=1)
+    rec, rec_source = load_counters(config)
+    roofline = roofline_record(cast_bytes, per_launch_ms, tst, cst, rec, rec_source)
+    log(f"[bench] roofline: {roofline}")
 
     cpu = None
     if not args.no_cpu:
@@ -179,18 +201,7 @@ def main():
             "live_per_depth": live,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_cast_refill (KD traversal + triangle intersection)",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "avg_launch_ms": round(per_launch_ms, 3),
-            "algorithmic_bytes_per_launch": int(cast_bytes / max(1, tst.cast_launches)),
-            "traffic_source": traffic["source"] if traffic else None,
-        },
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "batching": batching,
         "serving": serving,
@@ -199,26 +210,209 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(args):
-    """HBM bytes per cast launch counted by rocprofv3 on this workload
-    (profiles/cast_traffic_full1080p.json: FETCH_SIZE and WRITE_SIZE per
-    dispatch of the cast kernel, each scaled by what the counter reads for a
-    copy of known size), or None when no measurement of this configuration
-    is recorded."""
-    path = os.path.join(ROOT, "profiles", "cast_traffic_full1080p.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "cast_counters_full1080p.json")
+CAST_KERNEL = "k_cast_refill"
+SQ_SET = ("SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU "
+          "SQ_THREAD_CYCLES_VALU SQ_INSTS_SALU SQ_WAIT_INST_ANY")
+
+
+def device_source_sha():
+    """sha256 over the kernel sources and the C header, in name order: a
+    counter record applies only to the kernels it was measured on."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(ROOT, "hptracing_amd", "csrc", "device", "*")))
+    for f in files + [os.path.join(ROOT, "include", "hptracing.h")]:
+        h.update(os.path.basename(f).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def bench_config(args, runner):
+    """What a counter record must match: kernel sources, tuning, workload."""
+    return {"device_source_sha": device_source_sha(), "tuning": runner.get_tuning(),
+            "width": args.width, "height": args.height, "depth": args.depth,
+            "water": args.water, "supersample": not args.no_supersample,
+            "profile_steps": max(1, min(args.profile_steps, args.steps))}
+
+
+def load_counters(config):
+    """The counter record of profiles/cast_counters_full1080p.json, or
+    (None, reason) when it is missing or was measured on other kernel
+    sources, another tuning or another workload."""
+    if not os.path.exists(COUNTERS_FILE):
+        return None, "no counter record"
+    with open(COUNTERS_FILE) as f:
         rec = json.load(f)
-    same = (rec.get("width") == args.width and rec.get("height") == args.height
-            and rec.get("depth") == args.depth and rec.get("water") == args.water
-            and rec.get("supersample") == (not args.no_supersample)
-            and rec.get("profile_steps") == max(1, min(args.profile_steps, args.steps)))
-    if not same:
-        return None
-    return {"bytes_per_launch": int(rec["bytes_per_launch"]),
-            "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, {rec['kernel']}, "
-                      f"{rec['dispatches']} dispatches: {rec['command']}"}
+    for k, v in config.items():
+        if rec.get("config", {}).get(k) != v:
+            return None, f"counter record stale ({k} differs)"
+    return rec, "profiles/cast_counters_full1080p.json"
+
+
+def calibrate_copy():
+    """Plain device-to-device copies of known size for the byte counters."""
+    import torch
+    n = 1 << 28  # 256 Mi floats = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("[bench] copied 3 x 1 GiB")
+
+
+def _counter_rows(directory, kernel_substr):
+    """Per-dispatch sums of every counter of the kernels whose name holds
+    kernel_substr, from rocprofv3's counter_collection CSV under directory."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {directory}")
+    per = {}
+    for fn in files:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                if kernel_substr not in row["Kernel_Name"]:
+                    continue
+                key = (fn, row["Dispatch_Id"])
+                d = per.setdefault(key, {"kernel": row["Kernel_Name"]})
+                d[row["Counter_Name"]] = d.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    return list(per.values())
+
+
+def _avg(rows, name):
+    vals = [r[name] for r in rows if name in r]
+    if not vals:
+        raise SystemExit(f"counter {name} missing")
+    return sum(vals) / len(vals)
+
+
+def summarize_counters(directory):
+    """profiles/cast_counters_full1080p.json from the counter runs under
+    directory: bytes per launch (FETCH_SIZE x copy factor + WRITE_SIZE x
+    copy factor), the SQ set per dispatch, the kernel-trace average and the
+    derived issue figures."""
+    import csv
+    import glob
+    with open(os.path.join(directory, "bench_config.json")) as f:
+        cfg = json.load(f)
+    fetch = _counter_rows(os.path.join(directory, "fetch"), CAST_KERNEL)
+    write = _counter_rows(os.path.join(directory, "write"), CAST_KERNEL)
+    sq = _counter_rows(os.path.join(directory, "sq"), CAST_KERNEL)
+    cal_f = _counter_rows(os.path.join(directory, "cal_fetch"), "")
+    cal_w = _counter_rows(os.path.join(directory, "cal_write"), "")
+    copy_kib = float(1 << 20)  # one 1 GiB copy, in KiB
+    f_factor = copy_kib / max(r["FETCH_SIZE"] for r in cal_f)
+    w_factor = copy_kib / max(r["WRITE_SIZE"] for r in cal_w)
+    fetch_b = _avg(fetch, "FETCH_SIZE") * 1024 * f_factor
+    write_b = _avg(write, "WRITE_SIZE") * 1024 * w_factor
+    stats = glob.glob(os.path.join(directory, "trace", "**", "*kernel_stats.csv"), recursive=True)
+    avg_us = None
+    for fn in stats:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                if CAST_KERNEL in row["Name"]:
+                    avg_us = float(row["AverageNs"]) / 1e3
+    counters = {k: _avg(sq, k) for k in SQ_SET.split()}
+    l2_dir = os.path.join(directory, "l2")
+    l2_hit = None
+    if os.path.isdir(l2_dir):
+        l2 = _counter_rows(l2_dir, CAST_KERNEL)
+        hit, miss = _avg(l2, "TCC_HIT_sum"), _avg(l2, "TCC_MISS_sum")
+        counters.update({"TCC_HIT_sum": hit, "TCC_MISS_sum": miss})
+        l2_hit = hit / (hit + miss)
+    n_simd, n_se = 1024, 32
+    cycles = counters["SQ_BUSY_CYCLES"] / n_se
+    derived = {
+        "dispatch_cycles": cycles,
+        "clock_ghz": cycles / (avg_us * 1e3) if avg_us else None,
+        "valu_busy": counters["SQ_INSTS_VALU"] * 4 / (n_simd * cycles),
+        "active_lanes_of_64": counters["SQ_THREAD_CYCLES_VALU"] / counters["SQ_ACTIVE_INST_VALU"],
+        "waves_per_simd": counters["SQ_WAVE_CYCLES"] * 4 / (n_simd * cycles),
+        "salu_per_valu": counters["SQ_INSTS_SALU"] / counters["SQ_INSTS_VALU"],
+        "l2_hit": l2_hit,
+    }
+    rec = {
+        "kernel": sq[0]["kernel"],
+        "config": cfg,
+        "commands": {
+            "counters": "rocprofv3 --pmc <FETCH_SIZE | WRITE_SIZE | " + SQ_SET + " | "
+                        "TCC_HIT_sum TCC_MISS_sum> --output-format csv -d "
+                        "<DIR>/<fetch|write|sq|l2> -o run -- python bench.py "
+                        "--steps 4 --warmup 0 --profile-steps 4 --no-cpu --batch-views 0 "
+                        "--serve-requests 0 --cast-tris 0 --profile-record <DIR>",
+            "calibration": "rocprofv3 --pmc <FETCH_SIZE | WRITE_SIZE> --output-format csv "
+                           "-d <DIR>/<cal_fetch|cal_write> -o run -- python bench.py "
+                           "--calibrate-copy",
+            "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d <DIR>/trace -o run "
+                     "-- python bench.py --steps 4 --warmup 0 --profile-steps 4 --no-cpu "
+                     "--batch-views 0 --serve-requests 0 --cast-tris 0",
+        },
+        "dispatches": len(sq),
+        "avg_us": avg_us,
+        "fetch_factor": round(f_factor, 4), "write_factor": round(w_factor, 4),
+        "fetch_bytes_per_launch": int(fetch_b), "write_bytes_per_launch": int(write_b),
+        "bytes_per_launch": int(fetch_b + write_b),
+        "sq_per_dispatch": counters,
+        "derived": derived,
+    }
+    with open(COUNTERS_FILE, "w") as f:
+        json.dump(rec, f, indent=1)
+    log(f"[bench] wrote {COUNTERS_FILE}: {rec['bytes_per_launch'] / 1e9:.2f} GB per launch, "
+        f"VALU busy {derived['valu_busy']:.3f}, lanes {derived['active_lanes_of_64']:.1f}")
+
+
+def roofline_record(cast_bytes, per_launch_ms, tst, cst, rec, rec_source):
+    """The roofline object of the cast kernel: algorithmic bytes over the
+    HIP-event launch time against the HBM peak (`frac`), and, from the
+    counter record when it applies, the real traffic and the issue side."""
+    launches = max(1, tst.cast_launches)
+    algo = cast_bytes / launches
+    launch_s = per_launch_ms / 1e3
+    achieved = algo / launch_s / 1e9 if launch_s > 0 else 0.0
+    out = {
+        "bound": "hbm",
+        "kernel": "k_cast_refill (KD traversal + triangle intersection)",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(per_launch_ms, 3),
+        "algorithmic_bytes_per_launch": int(algo),
+        "counters_source": rec_source,
+    }
+    if rec is None:
+        return out
+    traffic = rec["bytes_per_launch"]
+    tgbs = traffic / launch_s / 1e9
+    d = rec["derived"]
+    sq = rec["sq_per_dispatch"]
+    lane_insts = sq["SQ_INSTS_VALU"] * d["active_lanes_of_64"]
+    visits = int(cst.node_visits) / launches
+    tests = int(cst.tri_tests) / launches
+    out.update({
+        "traffic": traffic,
+        "traffic_gbs": round(tgbs, 1),
+        "traffic_frac": round(tgbs / HBM_PEAK_GBS, 4),
+        "issue": {
+            "valu_busy": round(d["valu_busy"], 3),
+            "active_lanes_of_64": round(d["active_lanes_of_64"], 1),
+            "useful_lane_frac": round(d["valu_busy"] * d["active_lanes_of_64"] / 64, 3),
+            "waves_per_simd": round(d["waves_per_simd"], 2),
+            "l2_hit": round(d["l2_hit"], 3) if d.get("l2_hit") is not None else None,
+            "salu_per_valu": round(d["salu_per_valu"], 3),
+            "lane_valu_insts_per_node_visit_or_test": round(lane_insts / (visits + tests), 1),
+        },
+        "binding": (f"issue: VALU busy {d['valu_busy']:.2f} of cycles with "
+                    f"{d['active_lanes_of_64']:.1f} of 64 lanes active; HBM traffic "
+                    f"{tgbs / HBM_PEAK_GBS:.2f} of peak"),
+    })
+    return out
 
 
 def bench_batching(runner, n_views, depth):
@@ -379,6 +573,12 @@ def bench_cast_only(n_tris):
 
 
 def cpu_baseline(kd, view, spp, target_s, R):
+    """The oracle's float32 render (the reference kernels restated in C, one
+    path per thread) on the host cores, on a stratified sample of the same
+    view's camera rays: 32 bands of consecutive camera rays, one in the
+    middle of each 32nd of the frame, every band at the full spp and depth.
+    The sample grows until one run takes about target_s / 3; the last
+    sizing run is the warm-up, then the median of three timed runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -394,23 +594,38 @@ def cpu_baseline(kd, view, spp, target_s, R):
                      supersample_y=view["supersample_y"], background=view["background_color"],
                      sample=spp, depth=view["depth"], disable_diffuse=0, brightness=1.0, seed=1)
     s = O.Scene(kd)
-    # Camera rays spread over the image: a strided sample is not possible with
-    # a range, so take a band of rows in the middle of the frame.
-    n = max(256, cores * 16)
-    while True:
-        begin = R // 2 - n // 2
+    bands = 32
+    stratum = R // bands
+
+    def run(band):
+        rays = 0
         t0 = time.perf_counter()
-        _, rays = s.render_partial(ov, begin, begin + n, cores)
-        dt = time.perf_counter() - t0
-        if dt >= target_s * 0.5 or n >= R:
+        for k in range(bands):
+            begin = k * stratum + (stratum - band) // 2
+            _, r = s.render_partial(ov, begin, begin + band, cores)
+            rays += r
+        return rays, time.perf_counter() - t0
+
+    per_run = target_s / 3.0
+    band = max(8, cores)
+    while True:
+        rays, dt = run(band)              # sizing; the last one is the warm-up
+        if dt >= per_run * 0.6 or band >= stratum:
             break
-        n = min(R, int(n * max(2.0, target_s / max(dt, 1e-3))))
-    log(f"[bench] cpu baseline: {n} camera rays x {spp} spp, {rays} rays, {dt:.2f}s "
-        f"on {cores} threads")
-    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores,
+        band = min(stratum, int(band * max(2.0, per_run / max(dt, 1e-3))))
+    runs = [run(band) for _ in range(3)]
+    rates = sorted(r / t for r, t in runs)
+    median = rates[1]
+    n = band * bands
+    log(f"[bench] cpu baseline: {bands} bands x {band} camera rays x {spp} spp on {cores} "
+        f"threads, runs {[f'{r / t / 1e6:.3f}' for r, t in runs]} Mrays/s, median "
+        f"{median / 1e6:.3f}")
+    return {"value": round(median / 1e6, 4), "unit": "Mrays/s", "cores": cores,
             "kind": "port",
-            "sample": f"{n} camera rays from the middle rows of the same view, {spp} spp, "
-                      f"depth {view['depth']}: {rays} rays in {dt:.1f}s"}
+            "sample": f"{n} camera rays ({100.0 * n / R:.1f} % of the view) in {bands} bands "
+                      f"spread evenly over the frame, {spp} spp, depth {view['depth']}: "
+                      f"{runs[0][0]} rays per run; median of 3 runs after a warm-up "
+                      f"({', '.join(f'{r / t / 1e6:.3f}' for r, t in runs)} Mrays/s)"}
 
 
 if __name__ == "__main__":

@@ -81,6 +81,14 @@ class SceneDesc(C.Structure):
     ]
 
 
+class KdCounts(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int64), ("n_leaves", C.c_int64), ("n_leaf_entries", C.c_int64),
+        ("n_leaf_ints", C.c_int64), ("max_leaf", C.c_int64), ("max_depth", C.c_int32),
+        ("reserved", C.c_int32), ("seconds", C.c_double),
+    ]
+
+
 class SceneStats(C.Structure):
     _fields_ = [
         ("n_nodes", C.c_int64), ("n_leaf_entries", C.c_int64),
@@ -171,6 +179,7 @@ def host() -> C.CDLL:
         lib.hpt_host_load_obj.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(vp)]
         lib.hpt_host_from_arrays.argtypes = [vp, i64, vp, i64, vp, i64, C.c_int, C.POINTER(vp)]
         lib.hpt_host_describe.argtypes = [vp, C.POINTER(SceneDesc)]
+        lib.hpt_host_kd_count.argtypes = [vp, i64, vp, i64, C.c_int, C.POINTER(KdCounts)]
         lib.hpt_host_material_from_mtl.argtypes = [vp, vp, vp, vp, C.c_float, C.c_float,
                                                    C.c_float, C.POINTER(Material)]
         lib.hpt_host_free.argtypes = [vp]

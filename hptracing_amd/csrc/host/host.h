@@ -89,5 +89,13 @@ void scene_from_obj(const std::string &path, const std::string &mtl_basepath,
 void build_light_table(HostScene &scene);
 // KDTree ctor + getData (kdtree.cpp:235-292).
 void build_kdtree(HostScene &scene, int n_threads);
+// The size of the tree build_kdtree would make, without keeping it.
+struct KdCounts {
+    int64_t nodes = 0, leaves = 0, entries = 0, max_leaf = 0;
+    int max_depth = 0;
+    double seconds = 0;
+};
+KdCounts count_kdtree(const std::vector<float> &points, const std::vector<int32_t> &mesh,
+                      int n_threads);
 
 }  // namespace hpt

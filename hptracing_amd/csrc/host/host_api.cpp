@@ -84,6 +84,33 @@ int hpt_host_from_arrays(const float *points_xyz, int64_t n_points, const int32_
     });
 }
 
+int hpt_host_kd_count(const float *points_xyz, int64_t n_points, const int32_t *mesh,
+                      int64_t n_triangles, int n_threads, hpt_kd_counts *out) {
+    if (!points_xyz || !mesh || !out) return fail(HPT_ERR_INVALID, "null argument");
+    if (n_points <= 0 || n_triangles <= 0 || n_points > INT32_MAX || n_triangles > INT32_MAX)
+        return fail(HPT_ERR_INVALID, "array size out of range");
+    return guarded([&] {
+        std::vector<float> points(4 * n_points, 0.f);
+        for (int64_t i = 0; i < n_points; ++i)
+            for (int k = 0; k < 3; ++k) points[4 * i + k] = points_xyz[3 * i + k];
+        std::vector<int32_t> m(mesh, mesh + 4 * n_triangles);
+        for (int64_t i = 0; i < n_triangles; ++i)
+            for (int k = 0; k < 3; ++k)
+                if (m[4 * i + k] < 0 || m[4 * i + k] >= n_points)
+                    throw hpt::Error(HPT_ERR_SCENE, "triangle references a missing vertex");
+        const hpt::KdCounts c = hpt::count_kdtree(points, m, n_threads);
+        std::memset(out, 0, sizeof(*out));
+        out->n_nodes = c.nodes;
+        out->n_leaves = c.leaves;
+        out->n_leaf_entries = c.entries;
+        out->n_leaf_ints = c.leaves + c.entries;
+        out->max_leaf = c.max_leaf;
+        out->max_depth = c.max_depth;
+        out->seconds = c.seconds;
+        return HPT_OK;
+    });
+}
+
 int hpt_host_describe(const hpt_host_scene *scene, hpt_scene_desc *out) {
     if (!scene || !out) return fail(HPT_ERR_INVALID, "null argument");
     const hpt::HostScene &s = scene->s;

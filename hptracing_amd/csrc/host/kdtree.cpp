@@ -18,14 +18,16 @@
 //   * one pass of empty-leaf collapse, children first (:171-206);
 //   * pre-order numbering, sibling links on left children, leaf lists
 //     [count, ids...] (getData, :248-292).
-// Changed, without effect on the output: the per-node sorted bound arrays
-// are released once a node is split, and the two subtrees of large nodes
-// are built on separate threads.
+// Changed, without effect on the output: explicit stacks instead of the
+// recursion, the per-node sorted bound arrays released as soon as the
+// split is chosen, and the two subtrees of large nodes built on separate
+// threads.  count_kdtree walks the same splits keeping only counts.
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
 #include <cstring>
 #include <functional>
+#include <future>
 #include <thread>
 #include <unordered_map>
 
@@ -104,20 +106,28 @@ struct Builder {
         }
     }
 
-    // split (kdtree.cpp:93-169); `threads` is the number of extra threads
-    // this subtree may still start.
-    void split(Node &n, int threads) const {
-        if (n.geo.size() < 5) return;
+    // The split decision of split() (kdtree.cpp:93-130): false for a leaf
+    // (under 5 triangles, best cost above 3N, or a position on the border),
+    // else the axis of the first strictly smallest cost and its position.
+    bool choose_split(const Node &n, int &axis, float &pos) const {
+        if (n.geo.size() < 5) return false;
         const float no_split_cost = 3.f * (float)n.geo.size();
-        float pos[3], cost[3];
-        for (int d = 0; d < 3; ++d) best_split(n, d, pos[d], cost[d]);
+        float p[3], cost[3];
+        for (int d = 0; d < 3; ++d) best_split(n, d, p[d], cost[d]);
         int bd = 0;
         for (int d = 1; d < 3; ++d)
             if (cost[d] < cost[bd]) bd = d;
-        if (cost[bd] > no_split_cost) return;
-        const float p = pos[bd];
-        if (p <= n.box_start[bd] || p >= n.box_end[bd]) return;
+        if (cost[bd] > no_split_cost) return false;
+        if (p[bd] <= n.box_start[bd] || p[bd] >= n.box_end[bd]) return false;
+        axis = bd;
+        pos = p[bd];
+        return true;
+    }
 
+    // The children of a split (kdtree.cpp:132-156): the parent's box cut at
+    // `pos`; a triangle goes left unless all its vertices are > pos, right
+    // unless all are < pos.  The parent keeps its box and no triangles.
+    void partition(Node &n, int axis, float pos) const {
         n.left.reset(new Node());
         n.right.reset(new Node());
         Node &L = *n.left, &R = *n.right;
@@ -126,49 +136,194 @@ struct Builder {
         std::memcpy(R.box_start, n.box_start, sizeof(n.box_start));
         std::memcpy(L.box_end, n.box_end, sizeof(n.box_end));
         std::memcpy(R.box_end, n.box_end, sizeof(n.box_end));
-        L.box_end[bd] = p;
-        R.box_start[bd] = p;
+        L.box_end[axis] = pos;
+        R.box_start[axis] = pos;
         for (int32_t g : n.geo) {
             const int32_t *t = &mesh[4 * (int64_t)g];
-            const float a = coord(t[0], bd), b = coord(t[1], bd), c = coord(t[2], bd);
-            if (!(a > p && b > p && c > p)) L.geo.push_back(g);
-            if (!(a < p && b < p && c < p)) R.geo.push_back(g);
+            const float a = coord(t[0], axis), b = coord(t[1], axis), c = coord(t[2], axis);
+            if (!(a > pos && b > pos && c > pos)) L.geo.push_back(g);
+            if (!(a < pos && b < pos && c < pos)) R.geo.push_back(g);
         }
-        n.geo.clear();
-        n.geo.shrink_to_fit();
+        std::vector<int32_t>().swap(n.geo);
+    }
+
+    // Bounds, decision and (if split) children of one node; the sorted
+    // bounds are released before the children exist.
+    bool expand(Node &n) const {
+        calc_bounds(n);
+        int axis = 0;
+        float pos = 0.f;
+        const bool split = choose_split(n, axis, pos);
         for (int d = 0; d < 3; ++d) {
             std::vector<float>().swap(n.mins[d]);
             std::vector<float>().swap(n.maxs[d]);
         }
-        auto build = [this](Node &c, int t) {
-            calc_bounds(c);
-            split(c, t);
-            for (int d = 0; d < 3; ++d) {
-                std::vector<float>().swap(c.mins[d]);
-                std::vector<float>().swap(c.maxs[d]);
-            }
-        };
-        if (threads > 0 && L.geo.size() + R.geo.size() > 20000) {
-            const int rest = threads - 1;
-            std::thread worker(build, std::ref(R), rest / 2);
-            build(L, rest - rest / 2);
-            worker.join();
-        } else {
-            build(L, 0);
-            build(R, 0);
-        }
+        if (split) partition(n, axis, pos);
+        return split;
     }
 };
+
+// A node with more triangles than this in its two children hands the right
+// child's subtree to a new thread while the thread budget lasts.
+constexpr size_t kThreadMin = 20000;
+
+// Builds the subtree under `root` (box and triangles set) with an explicit
+// stack instead of the reference's recursion (split, kdtree.cpp:93-169):
+// nodes are expanded in the recursion's order (left subtree first), and
+// each node's result depends only on its own box and triangles, so the
+// tree is the same for every thread count.  `threads`: extra threads this
+// subtree may still start.
+void build_subtree(const Builder &b, Node *root, int threads) {
+    struct Item {
+        Node *n;
+        int threads;
+    };
+    std::vector<Item> stack{{root, threads}};
+    std::vector<std::thread> workers;
+    while (!stack.empty()) {
+        const Item it = stack.back();
+        stack.pop_back();
+        if (!b.expand(*it.n)) continue;
+        Node *L = it.n->left.get(), *R = it.n->right.get();
+        if (it.threads > 0 && L->geo.size() + R->geo.size() > kThreadMin) {
+            const int rest = it.threads - 1;
+            workers.emplace_back(build_subtree, std::cref(b), R, rest / 2);
+            stack.push_back({L, rest - rest / 2});
+        } else {
+            stack.push_back({R, 0});
+            stack.push_back({L, 0});
+        }
+    }
+    for (std::thread &w : workers) w.join();
+}
+
+// The root's box (setBoxSize, kdtree.cpp:36-41): the triangles' extremes
+// widened by 1e-3 (the front and back of the sorted bounds are the same
+// minimum and maximum).
+void fit_root_box(const Builder &b, Node &root) {
+    for (int d = 0; d < 3; ++d) {
+        float lo = FLT_MAX, hi = -FLT_MAX;
+        for (int32_t g : root.geo) {
+            const int32_t *t = &b.mesh[4 * (int64_t)g];
+            for (int k = 0; k < 3; ++k) {
+                lo = std::min(lo, b.coord(t[k], d));
+                hi = std::max(hi, b.coord(t[k], d));
+            }
+        }
+        root.box_start[d] = lo - 1e-3f;
+        root.box_end[d] = hi + 1e-3f;
+    }
+}
+
+// Counts of a subtree after the one-pass empty-leaf collapse.
+struct Counts {
+    int64_t nodes = 0, leaves = 0, entries = 0, max_leaf = 0;
+    int depth = 0;
+    bool empty_leaf = false;
+};
+
+Counts leaf_counts(size_t n) {
+    Counts c;
+    c.nodes = 1;
+    c.leaves = n ? 1 : 0;
+    c.entries = (int64_t)n;
+    c.max_leaf = (int64_t)n;
+    c.empty_leaf = n == 0;
+    return c;
+}
+
+// removeEmptyNode (kdtree.cpp:171-206) on a node whose children's subtrees
+// are already collapsed: a child that is an empty leaf makes the node take
+// the other child's contents (box, triangles, children).
+Counts combine(const Counts &l, const Counts &r) {
+    if (l.empty_leaf) return r;
+    if (r.empty_leaf) return l;
+    Counts c;
+    c.nodes = 1 + l.nodes + r.nodes;
+    c.leaves = l.leaves + r.leaves;
+    c.entries = l.entries + r.entries;
+    c.max_leaf = std::max(l.max_leaf, r.max_leaf);
+    c.depth = 1 + std::max(l.depth, r.depth);
+    return c;
+}
+
+// The counts of the tree build_subtree would make under `root`, without
+// keeping it: post-order over an explicit stack, each subtree released as
+// soon as it is counted, so memory stays at the triangle lists of the
+// nodes still pending on the current path.
+Counts count_subtree(const Builder &b, std::unique_ptr<Node> root, int threads) {
+    struct Frame {
+        std::unique_ptr<Node> n;       // stage 0: the node to expand
+        int threads = 0;
+        int stage = 0;                 // 1: left pending, 2: right pending
+        Counts left;
+        std::unique_ptr<Node> right;   // counted here after the left
+        std::future<Counts> right_async;
+    };
+    std::vector<Frame> stack;
+    stack.emplace_back();
+    stack.back().n = std::move(root);
+    stack.back().threads = threads;
+    Counts done;
+    bool have = false;
+    for (;;) {
+        if (have) {
+            if (stack.empty()) return done;
+            Frame &p = stack.back();
+            if (p.stage == 1) {
+                p.left = done;
+                have = false;
+                p.stage = 2;
+                if (p.right_async.valid()) {
+                    done = combine(p.left, p.right_async.get());
+                    have = true;
+                    stack.pop_back();
+                } else {
+                    std::unique_ptr<Node> r = std::move(p.right);
+                    stack.emplace_back();
+                    stack.back().n = std::move(r);
+                }
+            } else {
+                done = combine(p.left, done);
+                stack.pop_back();
+            }
+            continue;
+        }
+        Frame &f = stack.back();
+        Node &n = *f.n;
+        if (!b.expand(n)) {
+            done = leaf_counts(n.geo.size());
+            have = true;
+            stack.pop_back();
+            continue;
+        }
+        std::unique_ptr<Node> L = std::move(n.left), R = std::move(n.right);
+        L->parent = R->parent = nullptr;
+        f.n.reset();
+        f.stage = 1;
+        int child_threads = 0;
+        if (f.threads > 0 && L->geo.size() + R->geo.size() > kThreadMin) {
+            const int rest = f.threads - 1;
+            f.right_async = std::async(std::launch::async, count_subtree, std::cref(b),
+                                       std::move(R), rest / 2);
+            child_threads = rest - rest / 2;
+        } else {
+            f.right = std::move(R);
+        }
+        stack.emplace_back();
+        stack.back().n = std::move(L);
+        stack.back().threads = child_threads;
+    }
+}
 
 inline bool is_empty_leaf(const Node *n) {
     return n && n->geo.empty() && !n->left && !n->right;
 }
 
-// removeEmptyNode (kdtree.cpp:171-206).
-void remove_empty(Node &n) {
-    if (!n.left && !n.right) return;
-    remove_empty(*n.left);
-    remove_empty(*n.right);
+// removeEmptyNode (kdtree.cpp:171-206) at one node whose children were
+// already processed: a child that is an empty leaf makes the node take the
+// other child's box, triangles and children.
+void collapse(Node &n) {
     if (is_empty_leaf(n.left.get())) {
         std::unique_ptr<Node> keep = std::move(n.right);
         std::memcpy(n.box_start, keep->box_start, sizeof(n.box_start));
@@ -190,6 +345,24 @@ void remove_empty(Node &n) {
     if (n.left) n.left->parent = &n;
 }
 
+// The reference's one recursive pass, children first, over an explicit
+// stack: a node is collapsed after both of its subtrees.
+void remove_empty(Node *root) {
+    std::vector<std::pair<Node *, bool>> stack{{root, false}};
+    while (!stack.empty()) {
+        const auto [n, children_done] = stack.back();
+        stack.pop_back();
+        if (!n->left && !n->right) continue;
+        if (children_done) {
+            collapse(*n);
+            continue;
+        }
+        stack.push_back({n, true});
+        stack.push_back({n->right.get(), false});
+        stack.push_back({n->left.get(), false});
+    }
+}
+
 }  // namespace
 
 void build_kdtree(HostScene &scene, int n_threads) {
@@ -202,14 +375,9 @@ void build_kdtree(HostScene &scene, int n_threads) {
     std::unique_ptr<Node> root(new Node());
     root->geo.resize(nt);
     for (int64_t i = 0; i < nt; ++i) root->geo[i] = (int32_t)i;
-    b.calc_bounds(*root);
-    // setBoxSize (kdtree.cpp:36-41).
-    for (int d = 0; d < 3; ++d) {
-        root->box_start[d] = root->mins[d].front() - 1e-3f;
-        root->box_end[d] = root->maxs[d].back() + 1e-3f;
-    }
-    b.split(*root, n_threads - 1);
-    remove_empty(*root);
+    fit_root_box(b, *root);
+    build_subtree(b, root.get(), n_threads - 1);
+    remove_empty(root.get());
 
     // getData (kdtree.cpp:248-292): pre-order numbering, then one header per
     // node.  An explicit stack keeps deep trees off the call stack.
@@ -264,6 +432,28 @@ void build_kdtree(HostScene &scene, int n_threads) {
     }
     scene.build_seconds =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+KdCounts count_kdtree(const std::vector<float> &points, const std::vector<int32_t> &mesh,
+                      int n_threads) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t nt = (int64_t)mesh.size() / 4;
+    if (nt <= 0) throw Error(HPT_ERR_SCENE, "scene has no triangles");
+    if (n_threads <= 0) n_threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    Builder b{points, mesh};
+    std::unique_ptr<Node> root(new Node());
+    root->geo.resize(nt);
+    for (int64_t i = 0; i < nt; ++i) root->geo[i] = (int32_t)i;
+    fit_root_box(b, *root);
+    const Counts c = count_subtree(b, std::move(root), n_threads - 1);
+    KdCounts out;
+    out.nodes = c.nodes;
+    out.leaves = c.leaves;
+    out.entries = c.entries;
+    out.max_leaf = c.max_leaf;
+    out.max_depth = c.depth;
+    out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return out;
 }
 
 }  // namespace hpt

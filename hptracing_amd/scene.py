@@ -81,6 +81,19 @@ class KDTree:
             materials.ctypes.data, len(materials), n_threads, C.byref(h)))
         return cls(_handle=h)
 
+    @staticmethod
+    def count(points: np.ndarray, mesh: np.ndarray, n_threads: int = 0) -> dict:
+        """Size of the tree from_arrays would build over these triangles,
+        without building it (hpt_host_kd_count): nodes, non-empty leaves,
+        leaf entries, leaf ints, largest leaf, depth and seconds."""
+        points = np.ascontiguousarray(points, dtype=np.float32)
+        mesh = np.ascontiguousarray(mesh, dtype=np.int32)
+        out = N.KdCounts()
+        N.check_host(N.host().hpt_host_kd_count(points.ctypes.data, len(points),
+                                                mesh.ctypes.data, len(mesh), n_threads,
+                                                C.byref(out)))
+        return {k: getattr(out, k) for k, _ in N.KdCounts._fields_ if k != "reserved"}
+
     def _read_arrays(self) -> None:
         d = N.SceneDesc()
         N.check_host(N.host().hpt_host_describe(self._h, C.byref(d)))

@@ -113,6 +113,26 @@ int hpt_host_from_arrays(const float *points_xyz, int64_t n_points,
                          const hpt_material *materials, int64_t n_materials,
                          int n_threads, hpt_host_scene **out);
 
+/* Size of the KD tree hpt_host_from_arrays would build over these triangles
+ * (points xyz triples, mesh int4), found by the same splits and the same
+ * empty-leaf collapse without keeping the tree: each subtree is released
+ * once counted.  For sizing a scene before building it, where the tree may
+ * not fit in memory.  n_leaf_ints = n_leaves + n_leaf_entries is the length
+ * of getData()'s leaf list. */
+typedef struct {
+    int64_t n_nodes;
+    int64_t n_leaves;        /* non-empty leaves                  */
+    int64_t n_leaf_entries;  /* triangle ids over all leaf lists  */
+    int64_t n_leaf_ints;
+    int64_t max_leaf;        /* triangles in the largest leaf     */
+    int32_t max_depth;       /* edges from the root to the deepest leaf */
+    int32_t reserved;
+    double seconds;
+} hpt_kd_counts;
+int hpt_host_kd_count(const float *points_xyz, int64_t n_points,
+                      const int32_t *mesh, int64_t n_triangles, int n_threads,
+                      hpt_kd_counts *out);
+
 /* Fill `out` with pointers into the scene (valid until hpt_host_free):
  * the flattened tree of `KDTree::getData()` (kdtree.cpp:248-292) plus the
  * Scene vectors. */

@@ -239,3 +239,51 @@ def test_from_arrays_rejects_bad_indices():
     with pytest.raises(N.HptError) as e:
         KDTree.from_arrays(pts, mesh, np.array([material()]))
     assert e.value.code == N.HPT_ERR_SCENE
+
+
+def _tree_stats(kd):
+    nodes, leaf = kd.get_data()
+    depth = np.zeros(len(nodes), np.int64)
+    for i in range(1, len(nodes)):
+        depth[i] = depth[nodes[i]["parent"]] + 1
+    is_leaf = nodes["child"] < 0
+    full = is_leaf & (nodes["data"] >= 0)
+    counts = leaf[nodes["data"][full]]
+    return dict(n_nodes=len(nodes), n_leaves=int(full.sum()), n_leaf_entries=int(counts.sum()),
+                n_leaf_ints=len(leaf), max_leaf=int(counts.max()), max_depth=int(depth.max()))
+
+
+def _clustered_soup():
+    """A dense cluster in one corner of a box spanned by a few large
+    triangles: many empty-space cuts, so empty leaves are collapsed."""
+    from hptracing_amd import scenes
+    pts, mesh = scenes.soup(3000, seed=12)
+    pts = (pts * 0.05).astype(np.float32)
+    big = np.array([[0, 0, 0], [4, 0, 0], [0, 4, 0], [4, 4, 4], [0, 4, 4], [4, 0, 4]],
+                   np.float32)
+    base = len(pts)
+    mesh_big = np.array([[base, base + 1, base + 2, 0], [base + 3, base + 4, base + 5, 0]],
+                        np.int32)
+    return np.vstack([pts, big]), np.vstack([mesh, mesh_big])
+
+
+def test_kd_count_equals_the_built_tree(cornell, room, soup_small):
+    """hpt_host_kd_count walks the same splits without keeping the tree: its
+    nodes, leaves, entries, leaf ints, largest leaf and depth equal those of
+    the tree the builder returns, on scenes with and without empty-space
+    cuts (the one-pass empty-leaf collapse), and for any thread count."""
+    pts, mesh = _clustered_soup()
+    clustered = KDTree.from_arrays(pts, mesh, np.array([material(kd=(0.5, 0.5, 0.5))]))
+    for kd in (cornell, room, soup_small, clustered):
+        want = _tree_stats(kd)
+        for threads in (1, 4):
+            got = KDTree.count(kd.points[:, :3], kd.geometries, n_threads=threads)
+            assert {k: got[k] for k in want} == want, (kd.n_triangles, threads)
+
+
+def test_kd_count_rejects_bad_indices():
+    pts = np.zeros((3, 3), np.float32)
+    mesh = np.array([[0, 1, 3, 0]], np.int32)
+    with pytest.raises(N.HptError) as e:
+        KDTree.count(pts, mesh)
+    assert e.value.code == N.HPT_ERR_SCENE
