@@ -77,11 +77,23 @@ def build_boundary_client() -> str:
     return out
 
 
+def build_rcp_check() -> str:
+    """tests/cpp/rcp_exhaustive: the reciprocal of the intersection test
+    against the correctly rounded division over every float (GPU test)."""
+    src = os.path.join(ROOT, "tests", "cpp", "rcp_exhaustive.hip")
+    out = os.path.join(ROOT, "tests", "cpp", "rcp_exhaustive")
+    _run([HIPCC, *DEVICE_FLAGS, f"-I{os.path.join(PKG, 'csrc', 'device')}", f"-I{INCLUDE}", src,
+          "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> None:
     build_host()
     build_device()
     build_cli()
     build_boundary_client()
+    build_rcp_check()
 
 
 if __name__ == "__main__":

@@ -46,8 +46,10 @@ void load_obj(const std::string &path, const std::string &mtl_basepath,
 
 // ---------------------------------------------------------------- images
 // Decode a binary or ASCII PNM (P2/P3/P5/P6) file into planar float
-// channels, as CImg's native loader does.  Throws hpt::Error for any other
-// format (CImg hands those to an external converter).
+// channels, as CImg's native loader does.  decode_png / load_png give the
+// channels CImg receives for a PNG from ImageMagick's PNM output (gray: 1,
+// colour and palette: 3, alpha dropped, samples as stored); load_texture
+// picks the reader as CImg<float>(path) does.  All throw hpt::Error.
 struct Image {
     int w = 0, h = 0, c = 0;
     std::vector<float> data;  // planar: data[(ch * h + y) * w + x]
@@ -55,6 +57,9 @@ struct Image {
     float at(int x, int y, int ch) const { return data[((size_t)ch * h + y) * w + x]; }
 };
 Image load_pnm(const std::string &path);
+Image decode_png(const std::vector<uint8_t> &file, const std::string &name);
+Image load_png(const std::string &path);
+Image load_texture(const std::string &path);
 // CImg::normalize(0,255) then resize(512,512,1,3) with nearest neighbour,
 // then truncation to uint8 with alpha 0 (scene/base.cpp:98-110).
 void append_texture_layer(const Image &img, std::vector<uint8_t> &out);

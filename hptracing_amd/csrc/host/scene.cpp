@@ -84,12 +84,6 @@ bool pnm_int(std::istream &in, int &out) {
 }  // namespace
 
 Image load_pnm(const std::string &path) {
-    std::string ext;
-    size_t dot = path.rfind('.');
-    if (dot != std::string::npos) ext = path.substr(dot + 1);
-    for (auto &ch : ext) ch = (char)std::tolower((unsigned char)ch);
-    if (ext != "ppm" && ext != "pgm" && ext != "pnm")
-        throw Error(HPT_ERR_IO, "texture [" + path + "]: only PNM (P2/P3/P5/P6) images are decoded");
     std::ifstream in(path.c_str(), std::ios::binary);
     if (!in) throw Error(HPT_ERR_IO, "cannot open texture [" + path + "]");
     char m0 = 0, m1 = 0;
@@ -126,6 +120,32 @@ Image load_pnm(const std::string &path) {
                 img.at(x, y, ch) = (float)val;
             }
     return img;
+}
+
+// CImg<float>(path) as the reference builds it (no libpng or libjpeg): PNM
+// files by extension through CImg's own reader; every other file through
+// ImageMagick, which recognises the content (CImg.h:41305 load_other ->
+// :41063 load_imagemagick_external).  PNG and PNM content are decoded
+// here; anything else (JPEG and the rest of ImageMagick's formats) is an
+// error.
+Image load_texture(const std::string &path) {
+    std::string ext;
+    const size_t dot = path.rfind('.');
+    if (dot != std::string::npos) ext = path.substr(dot + 1);
+    for (auto &ch : ext) ch = (char)std::tolower((unsigned char)ch);
+    if (ext == "ppm" || ext == "pgm" || ext == "pnm") return load_pnm(path);
+    unsigned char head[8] = {};
+    {
+        std::ifstream in(path.c_str(), std::ios::binary);
+        if (!in) throw Error(HPT_ERR_IO, "cannot open texture [" + path + "]");
+        in.read(reinterpret_cast<char *>(head), 8);
+    }
+    if (head[0] == 0x89 && head[1] == 'P' && head[2] == 'N' && head[3] == 'G') return load_png(path);
+    if (head[0] == 'P' && (head[1] == '2' || head[1] == '3' || head[1] == '5' || head[1] == '6'))
+        return load_pnm(path);
+    if (head[0] == 0xFF && head[1] == 0xD8)
+        throw Error(HPT_ERR_IO, "texture [" + path + "]: JPEG textures are not decoded (PNG and PNM are)");
+    throw Error(HPT_ERR_IO, "texture [" + path + "]: only PNG and PNM images are decoded");
 }
 
 // CImg<float>::normalize(0, 255) (CImg.h:19093-19105), then
@@ -184,7 +204,7 @@ void scene_from_obj(const std::string &path, const std::string &mtl_basepath,
         scene.materials.push_back(x);
     }
     for (const auto &name : scene.texture_names)
-        append_texture_layer(load_pnm(mtl_basepath + name), scene.texture_data);
+        append_texture_layer(load_texture(mtl_basepath + name), scene.texture_data);
 
     for (const auto &shape : shapes) {
         const int64_t base = scene.n_points();

@@ -12,6 +12,10 @@
 #                                 and read_imagef a zero vector (-D options, the source
 #                                 is untouched).  Valid for untextured scenes only,
 #                                 where s1_run never reaches a texture read.
+#   oracle/_ref/ref_kernels_notex_contract.co  the untextured build with multiply-adds
+#                                 contracted into fused ones (-ffp-contract=fast, as
+#                                 NVIDIA's OpenCL compiler contracts by default), to
+#                                 show which image features depend on contraction
 #   oracle/_ref/libref_harness.so oracle/ref_harness.hip: loads those code objects and
 #                                 launches kdtree_intersect the way TraceRunner::run
 #                                 does (trace_runner.cpp:192-206), or the whole render
@@ -22,9 +26,9 @@
 # (double literals such as `/ 2.0` at kernel.cl:463 are float on a device
 # without fp64; clang rejects them otherwise), -cl-fp32-correctly-rounded-divide-sqrt
 # and -ffp-contract=off (IEEE division and no contraction, the arithmetic the
-# parity comparison is defined on), and no packed FP32 instructions
+# parity comparison is defined on; the _contract build excepted), and no packed FP32 instructions
 # (v_pk_add/mul/fma_f32 have given wrong results on these machines now and
-# then; the library's own device code is built the same way).  Only the two
+# then; the library's own device code is built the same way).  Only the
 # outputs above are written; no reference source is copied.
 set -euo pipefail
 REF=${HPT_REFERENCE:-/root/reference}
@@ -51,7 +55,15 @@ mv "$OUT/ref_kernels.co.tmp" "$OUT/ref_kernels.co"
     -D__read_only= -Dimage2d_array_t=int '-Dread_imagef(i,s,c)=((float4)(0.0f))' \
     "$REF/src/hp/cl_src/kernel.cl" -o "$OUT/ref_kernels_notex.co.tmp"
 mv "$OUT/ref_kernels_notex.co.tmp" "$OUT/ref_kernels_notex.co"
+"$CLANG" -x cl -cl-std=CL1.2 -target amdgcn-amd-amdhsa -mcpu=gfx950 \
+    --rocm-path=/opt/rocm -Xclang -finclude-default-header \
+    -include "$REF/src/hp/cl_src/types.h.cl" \
+    -cl-single-precision-constant -cl-fp32-correctly-rounded-divide-sqrt -ffp-contract=fast -O3 \
+    -Xclang -target-feature -Xclang -packed-fp32-ops \
+    -D__read_only= -Dimage2d_array_t=int '-Dread_imagef(i,s,c)=((float4)(0.0f))' \
+    "$REF/src/hp/cl_src/kernel.cl" -o "$OUT/ref_kernels_notex_contract.co.tmp"
+mv "$OUT/ref_kernels_notex_contract.co.tmp" "$OUT/ref_kernels_notex_contract.co"
 /opt/rocm/bin/hipcc -std=c++17 -O2 -fPIC -shared "$HERE/ref_harness.hip" \
     -o "$OUT/libref_harness.so.tmp" -Wno-unused-command-line-argument
 mv "$OUT/libref_harness.so.tmp" "$OUT/libref_harness.so"
-echo "build_ref: wrote $OUT/ref_kernels.co, $OUT/ref_kernels_notex.co and $OUT/libref_harness.so" >&2
+echo "build_ref: wrote $OUT/ref_kernels.co, $OUT/ref_kernels_notex.co, $OUT/ref_kernels_notex_contract.co and $OUT/libref_harness.so" >&2

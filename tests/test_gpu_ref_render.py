@@ -37,7 +37,7 @@ needs_ref = pytest.mark.skipif(not (os.path.exists(CO) and ref_abi.available()),
                                reason="reference kernels not built (oracle/build_ref.sh)")
 
 
-def ref_render(kd, view, spp, depth, seed, timed=False, primary=False):
+def ref_render(kd, view, spp, depth, seed, timed=False, primary=False, co=None):
     """results_mem of the reference's kernels for `view` (make_view's
     arguments), the S0 count each level's intersect ran over, with `timed`
     the host wall time of the render loop in seconds, and with `primary`
@@ -55,7 +55,7 @@ def ref_render(kd, view, spp, depth, seed, timed=False, primary=False):
     eye, up, right = f3(view["view_p"]), f3(view["top_dir"]), f3(view["right_dir"])
     bg = f3(view.get("background_color", (0.0, 0.0, 0.0)))
     arr = lambda a: (a.ctypes.data if len(a) else None, len(a))  # noqa: E731
-    rc = lib.ref_render(CO.encode(), *arr(kd.points), *arr(kd.normals), *arr(kd.texcoords),
+    rc = lib.ref_render((co or CO).encode(), *arr(kd.points), *arr(kd.normals), *arr(kd.texcoords),
                         *arr(kd.geometries), *arr(kd.materials), *arr(kd.lights),
                         *arr(kd.nodes), *arr(kd.leaf_data), eye.ctypes.data, up.ctypes.data,
                         right.ctypes.data, view["width"], view["height"], sx, sy, ssx, ssy,
@@ -310,3 +310,50 @@ def test_reference_kernels_speed_1080p(full_plain):
           f"{ref_live[1:].tolist()} ours {live[1:].tolist()}")
     rel = np.abs(live[1:] - ref_live[1:]) / ref_live[1:]
     assert (rel < 0.01).all(), rel
+
+
+CO_CONTRACT = os.path.join(REF_DIR, "ref_kernels_notex_contract.co")
+
+
+@needs_ref
+@pytest.mark.skipif(not os.path.exists(CO_CONTRACT), reason="contracted reference build absent")
+def test_reference_kernels_speckle_on_the_simple_cornell_box(cornell):
+    """The reference's simple Cornell box (tests/golden/cornell_box, a copy of
+    scene/simple-cornell-box) is 556 units wide: near x = 500 one float32 ulp
+    is 6e-5, above the 1e-5 step that moves a continuation ray off its
+    surface (kernel.cl:411), so whether a camera ray's hit point lets its
+    bounces leave the wall depends on the rounding of that point, and camera
+    rays are the same in every pass.  Without contraction -- the parity
+    arithmetic -- the reference's own kernels and this library show the same
+    fixed speckle: the same fraction of the red wall's camera rays stays dark
+    (below a quarter of the wall's median radiance).  Built with fused
+    multiply-adds (-ffp-contract=fast, as NVIDIA's OpenCL compiler does by
+    default, the build behind the published images) the reference's kernels
+    show far fewer dark rays."""
+    view = dict(view_p=(278.0, 273.0, -800.0), top_dir=(0.0, 1.0, 0.0), right_dir=(-1.0, 0.0, 0.0),
+                width=0.7, height=0.7, sample_x=64, sample_y=64, supersample_x=2,
+                supersample_y=2)
+    depth, spp = 10, 128
+    kd = cornell
+    red = [i for i, m in enumerate(kd.materials)
+           if m["diffuse"][0] > 0.4 and m["diffuse"][1] < 0.1]
+    assert red, "no red material"
+
+    def dark_fraction(acc, geo):
+        on = np.isin(geo[:, 3], red)
+        lum = acc[on].astype(np.float64).sum(axis=1) / spp
+        return float((lum < 0.25 * np.median(lum)).mean()), int(on.sum())
+
+    acc_off, _, geo_off = ref_render(kd, view, spp, depth, seed=61, primary=True)
+    acc_fma, _, geo_fma = ref_render(kd, view, spp, depth, seed=62, primary=True, co=CO_CONTRACT)
+    acc_gpu, _ = gpu_render(kd, view, spp, depth, seed=63)
+    geo_gpu = primary_geometry(kd, view)
+    off, n_off = dark_fraction(acc_off, geo_off)
+    fma, n_fma = dark_fraction(acc_fma, geo_fma)
+    gpu, n_gpu = dark_fraction(acc_gpu, geo_gpu)
+    print(f"\n[ref-render] simple Cornell box, {spp} spp: dark red-wall camera rays -- reference "
+          f"kernels without contraction {off:.4f} of {n_off}, with fused multiply-adds "
+          f"{fma:.4f} of {n_fma}, this library {gpu:.4f} of {n_gpu}")
+    sd = np.sqrt(2 * off * (1 - off) / n_off)
+    assert abs(gpu - off) <= 4 * sd + 0.01
+    assert fma < 0.5 * off
