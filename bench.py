@@ -85,6 +85,9 @@ def main():
                     help="only run the copy of known size for the byte counters")
     ap.add_argument("--summarize-counters", metavar="DIR",
                     help="only write the counter record from the rocprofv3 runs under DIR")
+    ap.add_argument("--serve-profile", action="store_true",
+                    help="only the serving burst for a kernel trace: a warm burst, a 1 s "
+                         "pause, the measured burst (default launch forms)")
     args = ap.parse_args()
 
     if args.summarize_counters:
@@ -92,6 +95,9 @@ def main():
         return
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.serve_profile:
+        serve_profile(args)
         return
 
     if args.gpus != 1:
@@ -364,6 +370,84 @@ def summarize_counters(directory):
         json.dump(rec, f, indent=1)
     log(f"[bench] wrote {COUNTERS_FILE}: {rec['bytes_per_launch'] / 1e9:.2f} GB per launch, "
         f"VALU busy {derived['valu_busy']:.3f}, lanes {derived['active_lanes_of_64']:.1f}")
+    write_stage_tables(directory, cfg, f_factor, w_factor, rec)
+
+
+def _kernel_stats(directory):
+    import csv
+    import glob
+    out = {}
+    for fn in glob.glob(os.path.join(directory, "trace", "**", "*kernel_stats.csv"), recursive=True):
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                out[row["Name"]] = (int(row["Calls"]), float(row["TotalDurationNs"]) / 1e6,
+                                    float(row["AverageNs"]) / 1e3, float(row["Percentage"]))
+    return out
+
+
+def _short(name):
+    return name.replace("void ", "").split("(")[0]
+
+
+def write_stage_tables(directory, cfg, f_factor, w_factor, rec):
+    """profiles/kernel_stats_full1080p.md and stage_counters_full1080p.md:
+    every kernel of the profile workload, from the same rocprofv3 runs."""
+    stats = _kernel_stats(directory)
+    per = {}
+    for key, sub in (("fetch", "fetch"), ("write", "write"), ("sq", "sq"), ("l2", "l2")):
+        for r in _counter_rows(os.path.join(directory, sub), ""):
+            d = per.setdefault(_short(r["kernel"]), {})
+            for c, v in r.items():
+                if c != "kernel":
+                    d.setdefault(c, []).append(v)
+    cmd_b = ("python bench.py --steps 4 --warmup 0 --profile-steps 4 --no-cpu --batch-views 0 "
+             "--serve-requests 0 --cast-tris 0")
+    t = cfg["tuning"]
+    lines = ["# Kernel time, full 1080p -x render (721 316 triangles, depth 8), 4 passes", "",
+             "Command (MI355X, ROCm 7.2):", "",
+             f"    rocprofv3 --kernel-trace --stats --output-format csv -d <DIR>/trace -o run -- {cmd_b}",
+             "", f"Kernel sources {cfg['device_source_sha']} (bench.py device_source_sha); tuning "
+             f"fetch {t['fetch']}, {t['cast_blocks_per_cu']} cast blocks per CU, refill "
+             f"{t['refill_threshold']}, node steps {t['node_steps']}.  The run holds the 4-pass "
+             "timed render and the 4-pass kernel-timed render (one launch sequence each: 1 primary "
+             "+ 7 bounce casts), then the 4-pass work-counting render, whose cast is the counting "
+             "variant `k_cast<true, ...>` (not the timed path).", "",
+             "| kernel | calls | total ms | average us | % |", "|---|---|---|---|---|"]
+    for name, (calls, tot, avg, pct) in sorted(stats.items(), key=lambda x: -x[1][1]):
+        lines.append(f"| `{_short(name)}` | {calls} | {tot:.2f} | {avg:.1f} | {pct:.2f} |")
+    with open(os.path.join(ROOT, "profiles", "kernel_stats_full1080p.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    n_simd, n_se = 1024, 32
+    rows = ["# Per-stage hardware counters, full 1080p -x render (721 316 triangles, depth 8)", "",
+            "Commands (MI355X, ROCm 7.2), one counter set per run, with the kernel trace of "
+            "kernel_stats_full1080p.md for durations and copies of known size for the byte "
+            "counters:", "", "    " + rec["commands"]["counters"], "    " + rec["commands"]["calibration"],
+            "", f"Per dispatch, averaged over each kernel's dispatches.  HBM bytes are FETCH_SIZE x "
+            f"{f_factor:.4f} + WRITE_SIZE x {w_factor:.4f} (what the counters read for a 1 GiB "
+            "copy); HBM TB/s divides them by the kernel-trace average.  VALU busy = SQ_INSTS_VALU "
+            "x 4 / (1024 SIMDs x SQ_BUSY_CYCLES / 32); lanes = SQ_THREAD_CYCLES_VALU / "
+            "SQ_ACTIVE_INST_VALU (of 64); waves per SIMD = SQ_WAVE_CYCLES x 4 / (1024 x cycles); "
+            "L2 hit = TCC_HIT / (TCC_HIT + TCC_MISS).  Kernel sources "
+            f"{cfg['device_source_sha']}.", "",
+            "| kernel | avg us | HBM read GB | HBM write GB | HBM TB/s | of 8 TB/s | L2 hit | "
+            "VALU busy | lanes | waves/SIMD |", "|---|---|---|---|---|---|---|---|---|---|"]
+    avg_of = {_short(k): v[2] for k, v in stats.items()}
+    for name, d in sorted(per.items(), key=lambda x: -avg_of.get(x[0], 0)):
+        if name not in avg_of or "SQ_BUSY_CYCLES" not in d or "FETCH_SIZE" not in d:
+            continue
+        a = lambda c: sum(d[c]) / len(d[c])  # noqa: E731
+        us = avg_of[name]
+        rd, wr = a("FETCH_SIZE") * 1024 * f_factor / 1e9, a("WRITE_SIZE") * 1024 * w_factor / 1e9
+        cyc = a("SQ_BUSY_CYCLES") / n_se
+        tbs = (rd + wr) / (us * 1e-6) / 1e3
+        hit = a("TCC_HIT_sum") / max(1.0, a("TCC_HIT_sum") + a("TCC_MISS_sum")) if "TCC_HIT_sum" in d else float("nan")
+        lanes = a("SQ_THREAD_CYCLES_VALU") / max(1.0, a("SQ_ACTIVE_INST_VALU"))
+        rows.append(f"| `{name}` | {us:.1f} | {rd:.3f} | {wr:.3f} | {tbs:.2f} | {tbs / 8:.2f} | "
+                    f"{hit:.3f} | {a('SQ_INSTS_VALU') * 4 / (n_simd * cyc):.2f} | {lanes:.1f} | "
+                    f"{a('SQ_WAVE_CYCLES') * 4 / (n_simd * cyc):.2f} |")
+    with open(os.path.join(ROOT, "profiles", "stage_counters_full1080p.md"), "w") as f:
+        f.write("\n".join(rows) + "\n")
+    log("[bench] wrote profiles/kernel_stats_full1080p.md and stage_counters_full1080p.md")
 
 
 def roofline_record(cast_bytes, per_launch_ms, tst, cst, rec, rec_source):
@@ -483,6 +567,15 @@ def bench_serving(args):
     log(f"[bench] one at a time: {len(views)} requests {seq_s:.2f}s {seq_mrays:.1f} Mrays/s")
     pat_s, _ = serving.one_at_a_time(runner, views, pass_at_a_time=True)
     log(f"[bench] one at a time, one sample pass per launch sequence: {pat_s:.2f}s")
+    # The same requests as one hpt_batch_render: every request in the same
+    # launches from the start, the best packing a server could reach on
+    # them, so seq_s / packed_s bounds served against one at a time.
+    t1 = time.perf_counter()
+    runner.render_batch(views)
+    packed_s = time.perf_counter() - t1
+    packed_mrays = int(runner.last_stats.rays_cast) / packed_s / 1e6
+    log(f"[bench] the mix as one batch: {packed_s:.2f}s {packed_mrays:.1f} Mrays/s "
+        f"(ceiling of served vs one at a time {seq_s / packed_s:.3f})")
     rates = []
     for rate in args.serve_rates:
         r = serving.serve(runner, views, rate, seed=11)
@@ -500,10 +593,32 @@ def bench_serving(args):
             "served_burst_s": round(burst.wall_s, 3),
             "served_burst_mrays_s": round(burst.mrays_s, 2),
             "served_vs_one_at_a_time": round(seq_s / burst.wall_s, 3),
+            "packed_batch_s": round(packed_s, 3), "packed_batch_mrays_s": round(packed_mrays, 2),
+            "served_ceiling_vs_one_at_a_time": round(seq_s / packed_s, 3),
             "one_at_a_time_pass_at_a_time_s": round(pat_s, 3),
             "served_vs_one_at_a_time_pass_at_a_time": round(pat_s / burst.wall_s, 3),
             "burst_p50_ms": round(burst.p50_ms, 2), "burst_p99_ms": round(burst.p99_ms, 2),
             "poisson": rates}
+
+
+def serve_profile(args):
+    """The serving burst alone, for `rocprofv3 --kernel-trace`: the 40-request
+    mix on the 2.2M-triangle scene submitted at once, twice, with a pause of
+    one second before the measured second burst (the trace's last cluster of
+    kernels)."""
+    from hptracing_amd import scenes, serving
+    from hptracing_amd.runner import TraceRunner
+    from hptracing_amd.scene import KDTree
+    path = scenes.cached(args.scene_dir, scenes.write_full, 0, water_side=1000)
+    kd = KDTree(path, os.path.dirname(path) + "/")
+    runner = TraceRunner(kd)
+    views = serving.request_views(args.serve_requests or 40, seed=3, depth=args.depth)
+    runner.render(views[0])
+    serving.serve(runner, views, 0.0, seed=11)
+    time.sleep(1.0)
+    b = serving.serve(runner, views, 0.0, seed=11)
+    log(f"[bench] serve profile: burst {b.wall_s:.3f}s, {b.mrays_s:.1f} Mrays/s, "
+        f"p50 {b.p50_ms:.0f} ms, p99 {b.p99_ms:.0f} ms, {b.macro_passes} macro passes")
 
 
 def bench_cast_only(n_tris):
@@ -576,7 +691,8 @@ def cpu_baseline(kd, view, spp, target_s, R):
     """The oracle's float32 render (the reference kernels restated in C, one
     path per thread) on the host cores, on a stratified sample of the same
     view's camera rays: 32 bands of consecutive camera rays, one in the
-    middle of each 32nd of the frame, every band at the full spp and depth.
+    middle of each 32nd of the frame, every band at the full spp and depth,
+    in one call of the oracle's render_rays.
     The sample grows until one run takes about target_s / 3; the last
     sizing run is the warm-up, then the median of three timed runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -593,17 +709,17 @@ def cpu_baseline(kd, view, spp, target_s, R):
                      sample_y=view["sample_y"], supersample_x=view["supersample_x"],
                      supersample_y=view["supersample_y"], background=view["background_color"],
                      sample=spp, depth=view["depth"], disable_diffuse=0, brightness=1.0, seed=1)
+    import numpy as np
     s = O.Scene(kd)
     bands = 32
     stratum = R // bands
+    accum = np.zeros(3 * R, np.float32)
 
     def run(band):
-        rays = 0
+        first = np.arange(bands, dtype=np.int64) * stratum + (stratum - band) // 2
+        picks = (first[:, None] + np.arange(band, dtype=np.int64)[None, :]).reshape(-1)
         t0 = time.perf_counter()
-        for k in range(bands):
-            begin = k * stratum + (stratum - band) // 2
-            _, r = s.render_partial(ov, begin, begin + band, cores)
-            rays += r
+        _, rays = s.render_rays(ov, picks, cores, accum=accum)
         return rays, time.perf_counter() - t0
 
     per_run = target_s / 3.0

@@ -252,16 +252,29 @@ uint64_t orc_path_seed(uint64_t seed, int64_t ray, int32_t pass) {
 }
 
 /* ----------------------------------------------------------- kernel.cl:559-613 */
+/* Direction of the camera ray (gx, gy) of set_viewdirs / set_viewdirs_vr. */
+static inline v3 pinhole_dir(v3 up, v3 right, float width, float height, int sx, int sy, int gx,
+                             int gy) {
+    v3 nd = cross(up, right);
+    nd = add(nd, scl((float)gy / (float)sy * height - height * 0.5f, up));
+    nd = add(nd, scl((float)gx / (float)sx * width - width * 0.5f, right));
+    return normalize3(nd);
+}
+
+static inline v3 vr_dir(int x_shift, int sample_y, int gx, int gy) {
+    float lon = (float)(gx + x_shift) / (float)(sample_y * 2) * PI_F * 2.f - PI_F;
+    float lat = (float)gy / (float)sample_y * PI_F - PI_F / 2.0f;
+    v3 d = mk(cosf(lat) * cosf(lon), sinf(lat), cosf(lat) * sinf(lon));
+    return normalize3(d);
+}
+
 void orc_set_viewdirs(const float view[3], const float up_[3], const float right_[3], float width,
                       float height, int sx, int sy, float *dirs) {
     (void)view;
     v3 up = ld3(up_), right = ld3(right_);
     for (int gy = 0; gy < sy; ++gy)
         for (int gx = 0; gx < sx; ++gx) {
-            v3 nd = cross(up, right);
-            nd = add(nd, scl((float)gy / (float)sy * height - height * 0.5f, up));
-            nd = add(nd, scl((float)gx / (float)sx * width - width * 0.5f, right));
-            v3 r = normalize3(nd);
+            v3 r = pinhole_dir(up, right, width, height, sx, sy, gx, gy);
             int64_t idx = (int64_t)sx * gy + gx;
             dirs[3 * idx + 0] = r.x;
             dirs[3 * idx + 1] = r.y;
@@ -273,10 +286,7 @@ void orc_set_viewdirs_vr(int x_shift, int sample_x, int sample_y, int grid_x, in
                          float *dirs) {
     for (int gy = 0; gy < grid_y; ++gy)
         for (int gx = 0; gx < grid_x; ++gx) {
-            float lon = (float)(gx + x_shift) / (float)(sample_y * 2) * PI_F * 2.f - PI_F;
-            float lat = (float)gy / (float)sample_y * PI_F - PI_F / 2.0f;
-            v3 d = mk(cosf(lat) * cosf(lon), sinf(lat), cosf(lat) * sinf(lon));
-            v3 r = normalize3(d);
+            v3 r = vr_dir(x_shift, sample_y, gx, gy);
             int64_t idx = (int64_t)sample_x * gy + gx;
             dirs[3 * idx + 0] = r.x;
             dirs[3 * idx + 1] = r.y;
@@ -708,6 +718,39 @@ int64_t orc_render_partial(const orc_scene *s, const orc_view *v, int64_t ray_be
         total += trace_paths(s, v, &init[r], id, t, accum, NULL);
     }
     free(init);
+    return total;
+}
+
+int64_t orc_render_rays(const orc_scene *s, const orc_view *v, const int64_t *rays, int64_t n,
+                        int n_threads, float *accum) {
+    const int sx = v->sample_x * v->supersample_x, sy = v->sample_y * v->supersample_y;
+    const int64_t R = (int64_t)sx * sy;
+    if (v->height < 0 && v->supersample_x != v->supersample_y) return -1;
+    const v3 up = ld3(v->up), right = ld3(v->right);
+    const int shift = (int)v->right[0] * v->supersample_x;
+    int64_t total = 0;
+    int nt = threads_or_all(n_threads);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) reduction(+ : total)
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t r = rays[k];
+        if (r < 0 || r >= R) continue;
+        const int gx = (int)(r % sx), gy = (int)(r / sx);
+        unit u;
+        memset(&u, 0, sizeof(u));
+        u.orig_id = (int32_t)r;
+        u.strength = mk(1, 1, 1);
+        u.start_p = ld3(v->view);
+        u.in_dir = v->height < 0 ? vr_dir(shift, sy, gx, gy)
+                                 : pinhole_dir(up, right, v->width, v->height, sx, sy, gx, gy);
+        float t;
+        int32_t id = kd_search(s, u.start_p, u.in_dir, &t, NULL);
+        total += 1;
+        if (id < 0) {
+            intersect_record(s, &u, accum, v->background, v->sample, NULL);
+            continue;
+        }
+        total += trace_paths(s, v, &u, id, t, accum, NULL);
+    }
     return total;
 }
 

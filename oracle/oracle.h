@@ -144,6 +144,12 @@ int orc_render_wavefront(const orc_scene *s, const orc_view *v, int seed_mode, f
  * bounded sample of the CPU baseline.  Returns rays cast. */
 int64_t orc_render_partial(const orc_scene *s, const orc_view *v, int64_t ray_begin,
                            int64_t ray_end, int n_threads, float *accum);
+/* Same render for the n camera rays listed in `rays` (any order, out-of-range
+ * entries skipped), each ray's direction computed alone: the stratified
+ * sample of the CPU baseline.  accum holds 3 floats per camera ray of the
+ * whole view.  Returns rays cast. */
+int64_t orc_render_rays(const orc_scene *s, const orc_view *v, const int64_t *rays, int64_t n,
+                        int n_threads, float *accum);
 
 uint64_t orc_path_seed(uint64_t seed, int64_t ray, int32_t pass);
 

@@ -76,6 +76,9 @@ def lib() -> C.CDLL:
         L.orc_render_partial.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcView), i64, i64,
                                          C.c_int, vp]
         L.orc_render_partial.restype = i64
+        L.orc_render_rays.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcView), vp, i64, C.c_int,
+                                      vp]
+        L.orc_render_rays.restype = i64
         L.orc_path_seed.argtypes = [C.c_uint64, i64, C.c_int32]
         L.orc_path_seed.restype = C.c_uint64
         L.orc_set_arithmetic.argtypes = [C.c_int]
@@ -154,6 +157,21 @@ class Scene:
         accum = np.zeros(R * 3, np.float32)
         n = lib().orc_render_partial(C.byref(self.c), C.byref(view), begin, end, n_threads,
                                      _p(accum))
+        return accum.reshape(-1, 3), int(n)
+
+
+    def render_rays(self, view: OrcView, rays, n_threads=0, accum=None):
+        """The render of the listed camera rays only; accum (3 floats per
+        camera ray of the view) is allocated unless given.  Returns (accum,
+        rays cast)."""
+        R = view.sample_x * view.supersample_x * view.sample_y * view.supersample_y
+        rays = np.ascontiguousarray(rays, np.int64)
+        if accum is None:
+            accum = np.zeros(R * 3, np.float32)
+        n = lib().orc_render_rays(C.byref(self.c), C.byref(view), _p(rays), len(rays), n_threads,
+                                  _p(accum))
+        if n < 0:
+            raise RuntimeError("orc_render_rays failed")
         return accum.reshape(-1, 3), int(n)
 
 

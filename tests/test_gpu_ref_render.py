@@ -319,17 +319,17 @@ CO_CONTRACT = os.path.join(REF_DIR, "ref_kernels_notex_contract.co")
 @pytest.mark.skipif(not os.path.exists(CO_CONTRACT), reason="contracted reference build absent")
 def test_reference_kernels_speckle_on_the_simple_cornell_box(cornell):
     """The reference's simple Cornell box (tests/golden/cornell_box, a copy of
-    scene/simple-cornell-box) is 556 units wide: near x = 500 one float32 ulp
-    is 6e-5, above the 1e-5 step that moves a continuation ray off its
-    surface (kernel.cl:411), so whether a camera ray's hit point lets its
-    bounces leave the wall depends on the rounding of that point, and camera
-    rays are the same in every pass.  Without contraction -- the parity
-    arithmetic -- the reference's own kernels and this library show the same
-    fixed speckle: the same fraction of the red wall's camera rays stays dark
-    (below a quarter of the wall's median radiance).  Built with fused
-    multiply-adds (-ffp-contract=fast, as NVIDIA's OpenCL compiler does by
-    default, the build behind the published images) the reference's kernels
-    show far fewer dark rays."""
+    scene/simple-cornell-box) is 556 units wide: near its far walls one
+    float32 ulp is 6e-5, above the 1e-5 step that moves a continuation ray
+    off its surface (kernel.cl:411, 447-553), so a hit point that rounds to
+    the far side of the (slanted) red wall starts its bounces behind the
+    wall, and camera rays are the same in every pass: a fixed fraction of
+    the wall's camera rays stays dark.  The reference's own kernels and this
+    library show the same fraction (the parity claim); the reference's
+    kernels built with fused multiply-adds (-ffp-contract=fast) show it as
+    well, so contraction is not what makes the published 5000-spp image of
+    this scene smooth.  (In a scratch build of the oracle with the step at
+    1e-3, camera rays of zero radiance drop from 12.5 % to 2.5 %.)"""
     view = dict(view_p=(278.0, 273.0, -800.0), top_dir=(0.0, 1.0, 0.0), right_dir=(-1.0, 0.0, 0.0),
                 width=0.7, height=0.7, sample_x=64, sample_y=64, supersample_x=2,
                 supersample_y=2)
@@ -355,5 +355,6 @@ def test_reference_kernels_speckle_on_the_simple_cornell_box(cornell):
           f"kernels without contraction {off:.4f} of {n_off}, with fused multiply-adds "
           f"{fma:.4f} of {n_fma}, this library {gpu:.4f} of {n_gpu}")
     sd = np.sqrt(2 * off * (1 - off) / n_off)
+    assert off > 0.2
     assert abs(gpu - off) <= 4 * sd + 0.01
-    assert fma < 0.5 * off
+    assert abs(fma - off) <= 0.1

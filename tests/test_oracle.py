@@ -282,3 +282,24 @@ def test_seed_schemes_statistically_equivalent(cornell):
     se = np.sqrt(np.var(outs[0], 0, ddof=1) / n_batches + np.var(outs[1], 0, ddof=1) / n_batches)
     ok = np.abs(m0 - m1) <= 4 * se + 1e-6
     assert ok.all(axis=1).mean() >= 0.99
+
+
+def test_render_rays_equals_the_full_render(room):
+    """orc_render_rays (the CPU baseline's stratified sample) computes each
+    listed camera ray's direction alone and traces its paths: the result on
+    those rays equals the full render's bit for bit, pinhole and
+    equirectangular, and unlisted rays stay zero."""
+    s = O.Scene(room)
+    for kw in [dict(view=(5, 5, -6), up=(0, 1, 0), right=(-1, 0, 0), width=1.0, height=1.0,
+                    sample_x=24, sample_y=20, supersample_x=2, supersample_y=2),
+               dict(view=(5, 5, 5), up=(0, 1, 0), right=(3, 0, 0), width=1.0, height=-1.0,
+                    sample_x=24, sample_y=12, supersample_x=2, supersample_y=2)]:
+        v = O.make_view(**kw, sample=3, depth=5, seed=17, background=(0.1, 0.2, 0.3))
+        full, _, _ = s.render(v)
+        R = len(full)
+        pick = np.random.default_rng(1).choice(R, R // 3, replace=False)
+        part, n = s.render_rays(v, pick)
+        assert n >= len(pick)
+        assert np.array_equal(part[pick].view(np.int32), full[pick].view(np.int32))
+        rest = np.setdiff1d(np.arange(R), pick)
+        assert (part[rest] == 0).all()
