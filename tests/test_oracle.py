@@ -303,3 +303,35 @@ def test_render_rays_equals_the_full_render(room):
         assert np.array_equal(part[pick].view(np.int32), full[pick].view(np.int32))
         rest = np.setdiff1d(np.arange(R), pick)
         assert (part[rest] == 0).all()
+
+
+def test_randf3_distribution():
+    """randf3 (kernel.cl:41-47) with alpha, beta = randf() * pi:
+    (cos b cos a, cos b sin a, sin b).  Its observable distribution -- all
+    the render can depend on -- is pinned here: z = sin(beta) has
+    P(z <= s) = (2 / pi) asin(s) on [0, 1]; the azimuth atan2(y, x) is
+    uniform on (-pi, pi] (cos b is symmetric about 0, so the upper
+    semicircle of (cos a, sin a) fills the circle) and independent of the
+    radius |cos b| = sqrt(1 - z^2).  Which of alpha and beta is drawn first,
+    and whether x takes cos a or sin a, leave this distribution unchanged
+    (alpha and beta are independent and identically distributed), so no
+    image can tell them apart.  Kolmogorov-Smirnov at the 1e-4 level on
+    200 000 draws; a z of cos(beta) or of a plain randf(), or an azimuth
+    confined to a half plane, fails."""
+    from scipy import stats
+    v, _ = O.rand_sequence(2024, 200_000, "randf3")
+    v = np.asarray(v, np.float64)
+    z = v[:, 2]
+    assert stats.kstest(z, lambda s: 2 / np.pi * np.arcsin(np.clip(s, 0, 1))).pvalue > 1e-4
+    phi = np.arctan2(v[:, 1], v[:, 0])
+    assert stats.kstest(phi, stats.uniform(-np.pi, 2 * np.pi).cdf).pvalue > 1e-4
+    # azimuth independent of the radius: same law in the inner and outer halves
+    r = np.hypot(v[:, 0], v[:, 1])
+    inner = r < np.median(r)
+    assert stats.ks_2samp(phi[inner], phi[~inner]).pvalue > 1e-4
+    # the tests have power: plausible mistakes fail them
+    u = np.random.default_rng(0).uniform(0, np.pi, (2, 200_000))
+    assert stats.kstest(np.cos(u[1]), lambda s: 2 / np.pi * np.arcsin(np.clip(s, 0, 1))).pvalue < 1e-4
+    assert stats.kstest(u[1] / np.pi, lambda s: 2 / np.pi * np.arcsin(np.clip(s, 0, 1))).pvalue < 1e-4
+    half = np.arctan2(np.sin(u[0]), np.cos(u[0]))      # |cos b| instead of cos b: a half plane
+    assert stats.kstest(half, stats.uniform(-np.pi, 2 * np.pi).cdf).pvalue < 1e-4
