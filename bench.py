@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--batch-views", type=int, default=64, help="0 skips the batching leg")
     ap.add_argument("--serve-requests", type=int, default=40, help="0 skips the serving leg")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[2.0, 8.0])
+    ap.add_argument("--literal-steps", type=int, default=None,
+                    help="passes of the literal-arithmetic comparison render (default: --steps; "
+                         "0 skips it)")
     ap.add_argument("--cast-tris", type=int, default=200_000,
                     help="triangles of the cast-only soup (0 skips the cast-only leg)")
     ap.add_argument("--profile-record", metavar="DIR",
@@ -145,6 +148,24 @@ def main():
     log(f"[bench] timed render: {args.steps} passes, {rays} rays, {elapsed:.3f}s, "
         f"{value:.1f} Mrays/s; live per depth {live}")
 
+    # The same render with the other arithmetic of the intersection test
+    # (every product and sum rounded, as kernel.cl writes dot/cross), for
+    # comparison only; the default is restored before anything else runs.
+    literal = None
+    lit_steps = args.steps if args.literal_steps is None else args.literal_steps
+    if lit_steps > 0:
+        runner.set_arithmetic(False)
+        torch.cuda.synchronize()
+        t_lit = time.perf_counter()
+        runner.render(make_view(**view, sample=lit_steps, seed=1))
+        torch.cuda.synchronize()
+        lit_elapsed = time.perf_counter() - t_lit
+        lit_rays = int(runner.last_stats.rays_cast)
+        runner.set_arithmetic(True)
+        literal = {"steps": lit_steps, "mrays_s": round(lit_rays / lit_elapsed / 1e6, 3),
+                   "ms_per_step": round(lit_elapsed * 1e3 / lit_steps, 3)}
+        log(f"[bench] literal arithmetic: {lit_elapsed:.3f}s, {literal['mrays_s']:.1f} Mrays/s")
+
     # Kernel time and algorithmic work of the same launches.
     pk = max(1, min(args.profile_steps, args.steps))
     runner.render(make_view(**view, sample=pk, seed=1), time_kernels=True)
@@ -205,8 +226,12 @@ def main():
             "kd_nodes": len(kd.nodes),
             "rays_cast": rays,
             "live_per_depth": live,
+            "arithmetic": "rocm_opencl: dot and cross of the intersection test as ROCm's "
+                          "OpenCL library evaluates them (hit triangles and distances equal "
+                          "the reference kernel's bit for bit)",
             "parallelism": "single GPU",
         },
+        "literal_arithmetic": literal,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batching": batching,
@@ -350,13 +375,13 @@ def summarize_counters(directory):
                         "TCC_HIT_sum TCC_MISS_sum> --output-format csv -d "
                         "<DIR>/<fetch|write|sq|l2> -o run -- python bench.py "
                         "--steps 4 --warmup 0 --profile-steps 4 --no-cpu --batch-views 0 "
-                        "--serve-requests 0 --cast-tris 0 --profile-record <DIR>",
+                        "--serve-requests 0 --cast-tris 0 --literal-steps 0 --profile-record <DIR>",
             "calibration": "rocprofv3 --pmc <FETCH_SIZE | WRITE_SIZE> --output-format csv "
                            "-d <DIR>/<cal_fetch|cal_write> -o run -- python bench.py "
                            "--calibrate-copy",
             "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d <DIR>/trace -o run "
                      "-- python bench.py --steps 4 --warmup 0 --profile-steps 4 --no-cpu "
-                     "--batch-views 0 --serve-requests 0 --cast-tris 0",
+                     "--batch-views 0 --serve-requests 0 --cast-tris 0 --literal-steps 0",
         },
         "dispatches": len(sq),
         "avg_us": avg_us,
@@ -401,7 +426,7 @@ def write_stage_tables(directory, cfg, f_factor, w_factor, rec):
                 if c != "kernel":
                     d.setdefault(c, []).append(v)
     cmd_b = ("python bench.py --steps 4 --warmup 0 --profile-steps 4 --no-cpu --batch-views 0 "
-             "--serve-requests 0 --cast-tris 0")
+             "--serve-requests 0 --cast-tris 0 --literal-steps 0")
     t = cfg["tuning"]
     lines = ["# Kernel time, full 1080p -x render (721 316 triangles, depth 8), 4 passes", "",
              "Command (MI355X, ROCm 7.2):", "",
@@ -492,9 +517,11 @@ def roofline_record(cast_bytes, per_launch_ms, tst, cst, rec, rec_source):
             "salu_per_valu": round(d["salu_per_valu"], 3),
             "lane_valu_insts_per_node_visit_or_test": round(lane_insts / (visits + tests), 1),
         },
-        "binding": (f"issue: VALU busy {d['valu_busy']:.2f} of cycles with "
-                    f"{d['active_lanes_of_64']:.1f} of 64 lanes active; HBM traffic "
-                    f"{tgbs / HBM_PEAK_GBS:.2f} of peak"),
+        "binding": (f"latency of dependent random fetches below the L2 (HBM traffic "
+                    f"{tgbs / HBM_PEAK_GBS:.2f} of peak in 64- and 128-byte random reads), "
+                    f"not VALU issue (busy {d['valu_busy']:.2f} of cycles, "
+                    f"{d['active_lanes_of_64']:.1f} of 64 lanes active; cutting VALU "
+                    f"instructions 0.8 % cut the launch 0.9 %)"),
     })
     return out
 
@@ -675,6 +702,10 @@ def bench_cast_only(n_tris):
         f"{ms:.2f} ms, {n / ms / 1e3:.1f} Mrays/s, {algo / n:.0f} algorithmic B/ray, "
         f"{gbs:.0f} GB/s; autotuned {n / ms_tuned / 1e3:.1f} Mrays/s; setup {time.time() - t0:.1f}s")
     return {"workload": f"hpt_cast, {n} rays through a {kd.n_triangles}-triangle soup",
+            "stand_in_for": ("the 1 000 000-triangle soup: its exact KD tree holds "
+                             "9 321 406 973 leaf ints (hpt_host_kd_count), past the int32 "
+                             "offsets of the reference's node format; 600 000 triangles "
+                             "(1 664 622 455) is the largest size that fits"),
             "triangles": kd.n_triangles, "rays": n, "ms": round(ms, 3),
             "mrays_s": round(n / ms / 1e3, 2),
             "node_visits_per_ray": round(int(c[0]) / n, 1),

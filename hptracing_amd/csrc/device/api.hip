@@ -232,7 +232,7 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.arith_ocl = 1;
         ds.refill = 48;
         ds.node_steps = 24;
-        ds.fetch = 157;
+        ds.fetch = 445;
         ds.walk_min = 0;
 
         int cus = 0;

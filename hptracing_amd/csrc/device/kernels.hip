@@ -403,11 +403,12 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
 // results are those of trace_ray.
 //
 // Refinements, none of which changes a result:
-//  PAIRS  nodes are fetched two at a time (idx and idx + 1, 64 contiguous
-//         bytes).  After a leaf, and after an interior node whose box is
-//         hit, the walk moves to idx + 1, so the second node is usually the
-//         next one visited and its box test needs no further memory trip;
-//         only a missed interior node (a jump to its escape) waits.
+//  WIN    nodes are fetched WIN at a time (idx .. idx + WIN - 1, contiguous;
+//         WIN = 2 is the node pair).  After a leaf, and after an interior
+//         node whose box is hit, the walk moves to idx + 1, so the next
+//         nodes are usually the next ones visited and their box tests need
+//         no further memory trip; only a missed interior node (a jump to its
+//         escape) waits, unless its escape still lies in the window.
 //  PREF   a leaf's triangle records are fetched one triangle ahead, so the
 //         loads of the next record are in flight during the current test.
 //  Q > 0  FLAT form: the leaf phase is spread over the whole wave.  A lane
@@ -423,7 +424,7 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
 //         (keyed_leaf_round) and testers read rays from LDS.
 //  IND    (FLAT only) leaf rounds read the records once per triangle
 //         through the leaf entries' triangle ids (tri_record).
-template <bool PATHS, bool FAST, bool PAIRS, bool PREF, int Q, bool KEYED, bool IND>
+template <bool PATHS, bool FAST, int WIN, bool PREF, int Q, bool KEYED, bool IND>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
                                             const float *dir3, const float4 *org4,
                                             const float4 *dir4, int64_t n, WorkCounter *wc,
@@ -552,7 +553,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
         // `walk_min` lanes walk.  The loop is written once per visit form
         // (the form is fixed for the round), so that the loop-carried
         // walk state needs no merge of two code paths on every step.
-        if (all_rcp && !PAIRS) {
+        if (all_rcp && WIN < 2) {
             for (int k = 0; k < node_steps; ++k) {
                 const bool walk = ray >= 0 && !done && pending < QN;
                 const uint64_t wm = __ballot(walk);
@@ -569,15 +570,25 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                 const uint64_t wm = __ballot(walk);
                 if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
                 if (walk) {
-                    const float4 lo = sc.nodes[2 * idx];
-                    const float4 hi = sc.nodes[2 * idx + 1];
-                    if (PAIRS) {
-                        const int nxt = idx + 1 < nn ? idx + 1 : idx;
-                        const float4 lo1 = sc.nodes[2 * nxt];
-                        const float4 hi1 = sc.nodes[2 * nxt + 1];
-                        visit(lo, hi);
-                        if (!done && pending < QN && idx == nxt) visit(lo1, hi1);
+                    if (WIN > 1) {
+                        // The walk only moves forward, so every node of the
+                        // window that it reaches is visited in order; the
+                        // loads past the last node repeat that node.
+                        float4 lo[WIN], hi[WIN];
+#pragma unroll
+                        for (int j = 0; j < WIN; ++j) {
+                            const int q = idx + j < nn ? idx + j : nn - 1;
+                            lo[j] = sc.nodes[2 * q];
+                            hi[j] = sc.nodes[2 * q + 1];
+                        }
+                        const int first = idx;
+                        visit(lo[0], hi[0]);
+#pragma unroll
+                        for (int j = 1; j < WIN; ++j)
+                            if (!done && pending < QN && idx == first + j) visit(lo[j], hi[j]);
                     } else {
+                        const float4 lo = sc.nodes[2 * idx];
+                        const float4 hi = sc.nodes[2 * idx + 1];
                         visit(lo, hi);
                     }
                 }
@@ -630,7 +641,8 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
 // bits: 1 node pairs, 2 triangle prefetch, 4 FLAT, and with 4 the leaf
 // queue depth 1 << ((FETCH >> 3) & 3); bit 32 compiles the form for six
 // waves per SIMD (80 VGPRs) instead of eight (64); bit 64 selects the keyed
-// leaf-round fold, bit 128 the indirect (per-triangle) records.
+// leaf-round fold, bit 128 the indirect (per-triangle) records; with bit 1,
+// bits 8-9 widen the node window from 2 to 2 + ((FETCH >> 8) & 3).
 // The keyed forms hold 21 KB (queue of 4) or 29 KB (queue of 8) of LDS per
 // block, so at most 7 or 5 blocks fit in a CU's 160 KB.
 constexpr int cast_waves_per_eu(int fetch) {
@@ -645,7 +657,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     CountRef n_dev, int64_t n_host, WorkCounter *wc, int *out_id, float *out_t) {
     const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
     constexpr int Q = (FETCH & 4) ? (1 << ((FETCH >> 3) & 3)) : 0;
-    cast_refill<PATHS, FAST, (FETCH & 1) != 0, (FETCH & 6) == 2, Q, (FETCH & 64) != 0,
+    constexpr int WIN = (FETCH & 1) ? 2 + ((FETCH >> 8) & 3) : 1;
+    cast_refill<PATHS, FAST, WIN, (FETCH & 6) == 2, Q, (FETCH & 64) != 0,
                 (FETCH & 128) != 0>(sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
     wave_done(wc);
 }
@@ -1175,6 +1188,10 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
     case 189: HPT_REFILL(P, F, 189); break;                                                      \
     case 220: HPT_REFILL(P, F, 220); break;                                                      \
     case 252: HPT_REFILL(P, F, 252); break;                                                      \
+    case 413: HPT_REFILL(P, F, 413); break;                                                      \
+    case 445: HPT_REFILL(P, F, 445); break;                                                      \
+    case 669: HPT_REFILL(P, F, 669); break;                                                      \
+    case 701: HPT_REFILL(P, F, 701); break;                                                      \
     default: HPT_REFILL(P, F, 0); break;                                                         \
     }
         if (paths) { if (fast) { HPT_REFILL_M(true, true) } else { HPT_REFILL_M(true, false) } }

@@ -46,7 +46,8 @@ inline bool valid_fetch(int f) {
     switch (f) {
     case 0: case 1: case 2: case 3: case 4: case 5: case 12: case 13: case 20: case 21:
     case 28: case 29: case 52: case 60: case 84: case 92: case 116: case 124: case 148:
-    case 149: case 156: case 157: case 189: case 220: case 252:
+    case 149: case 156: case 157: case 189: case 220: case 252: case 413: case 445:
+    case 669: case 701:
         return true;
     default:
         return false;
