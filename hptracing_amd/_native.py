@@ -180,6 +180,8 @@ def host() -> C.CDLL:
         lib.hpt_host_from_arrays.argtypes = [vp, i64, vp, i64, vp, i64, C.c_int, C.POINTER(vp)]
         lib.hpt_host_describe.argtypes = [vp, C.POINTER(SceneDesc)]
         lib.hpt_host_kd_count.argtypes = [vp, i64, vp, i64, C.c_int, C.POINTER(KdCounts)]
+        lib.hpt_host_read_image.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32),
+                                            C.POINTER(i32), vp, i64]
         lib.hpt_host_material_from_mtl.argtypes = [vp, vp, vp, vp, C.c_float, C.c_float,
                                                    C.c_float, C.POINTER(Material)]
         lib.hpt_host_free.argtypes = [vp]

@@ -48,8 +48,10 @@ void load_obj(const std::string &path, const std::string &mtl_basepath,
 // Decode a binary or ASCII PNM (P2/P3/P5/P6) file into planar float
 // channels, as CImg's native loader does.  decode_png / load_png give the
 // channels CImg receives for a PNG from ImageMagick's PNM output (gray: 1,
-// colour and palette: 3, alpha dropped, samples as stored); load_texture
-// picks the reader as CImg<float>(path) does.  All throw hpt::Error.
+// colour and palette: 3, alpha dropped, samples as stored); decode_jpeg /
+// load_jpeg give the samples of libjpeg's default decode (gray: 1, colour:
+// 3) that ImageMagick prints for a JPEG; load_texture picks the reader as
+// CImg<float>(path) does.  All throw hpt::Error.
 struct Image {
     int w = 0, h = 0, c = 0;
     std::vector<float> data;  // planar: data[(ch * h + y) * w + x]
@@ -59,6 +61,8 @@ struct Image {
 Image load_pnm(const std::string &path);
 Image decode_png(const std::vector<uint8_t> &file, const std::string &name);
 Image load_png(const std::string &path);
+Image decode_jpeg(const std::vector<uint8_t> &file, const std::string &name);
+Image load_jpeg(const std::string &path);
 Image load_texture(const std::string &path);
 // CImg::normalize(0,255) then resize(512,512,1,3) with nearest neighbour,
 // then truncation to uint8 with alpha 0 (scene/base.cpp:98-110).

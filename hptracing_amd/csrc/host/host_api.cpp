@@ -111,6 +111,23 @@ int hpt_host_kd_count(const float *points_xyz, int64_t n_points, const int32_t *
     });
 }
 
+int hpt_host_read_image(const char *path, int32_t *w, int32_t *h, int32_t *c, float *out,
+                        int64_t capacity) {
+    if (!path || !w || !h || !c) return fail(HPT_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const hpt::Image img = hpt::load_texture(path);
+        *w = img.w;
+        *h = img.h;
+        *c = img.c;
+        if (out) {
+            if (capacity < (int64_t)img.data.size())
+                throw hpt::Error(HPT_ERR_INVALID, "image buffer too small");
+            std::memcpy(out, img.data.data(), img.data.size() * sizeof(float));
+        }
+        return HPT_OK;
+    });
+}
+
 int hpt_host_describe(const hpt_host_scene *scene, hpt_scene_desc *out) {
     if (!scene || !out) return fail(HPT_ERR_INVALID, "null argument");
     const hpt::HostScene &s = scene->s;

@@ -143,9 +143,8 @@ Image load_texture(const std::string &path) {
     if (head[0] == 0x89 && head[1] == 'P' && head[2] == 'N' && head[3] == 'G') return load_png(path);
     if (head[0] == 'P' && (head[1] == '2' || head[1] == '3' || head[1] == '5' || head[1] == '6'))
         return load_pnm(path);
-    if (head[0] == 0xFF && head[1] == 0xD8)
-        throw Error(HPT_ERR_IO, "texture [" + path + "]: JPEG textures are not decoded (PNG and PNM are)");
-    throw Error(HPT_ERR_IO, "texture [" + path + "]: only PNG and PNM images are decoded");
+    if (head[0] == 0xFF && head[1] == 0xD8 && head[2] == 0xFF) return load_jpeg(path);
+    throw Error(HPT_ERR_IO, "texture [" + path + "]: only PNG, JPEG and PNM images are decoded");
 }
 
 // CImg<float>::normalize(0, 255) (CImg.h:19093-19105), then

@@ -43,6 +43,20 @@ def _array(ptr, count, dtype, width=1):
     return a.reshape(count, width) if width > 1 else a
 
 
+def read_image(path: str) -> np.ndarray:
+    """An image file as the texture pipeline reads it (CImg<float>(path),
+    scene/base.cpp:100): (height, width, channels) float32 samples, gray
+    images with one channel and colour images with three."""
+    w, h, c = C.c_int32(), C.c_int32(), C.c_int32()
+    lib = N.host()
+    N.check_host(lib.hpt_host_read_image(path.encode(), C.byref(w), C.byref(h), C.byref(c),
+                                         None, 0))
+    out = np.empty((c.value, h.value, w.value), np.float32)
+    N.check_host(lib.hpt_host_read_image(path.encode(), C.byref(w), C.byref(h), C.byref(c),
+                                         out.ctypes.data, out.size))
+    return np.ascontiguousarray(out.transpose(1, 2, 0))
+
+
 def material(ka=(0, 0, 0), kd=(0, 0, 0), ks=(0, 0, 0), ke=(0, 0, 0), ior=1.0,
              dissolve=1.0, shininess=1.0) -> np.void:
     """One material record with the reference's branch probabilities

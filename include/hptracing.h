@@ -133,6 +133,15 @@ int hpt_host_kd_count(const float *points_xyz, int64_t n_points,
                       const int32_t *mesh, int64_t n_triangles, int n_threads,
                       hpt_kd_counts *out);
 
+/* Read an image file as the texture pipeline reads it (CImg<float>(path),
+ * scene/base.cpp:100): PNM natively, PNG and JPEG as ImageMagick's PNM
+ * output gives them (gray one channel, colour three).  Writes the size to
+ * *w, *h, *c and, when `out` is not NULL and `capacity` >= w * h * c, the
+ * samples as planar floats (out[(ch * h + y) * w + x]); with `out` NULL
+ * only the size is written.  HPT_ERR_INVALID when `out` is too small. */
+int hpt_host_read_image(const char *path, int32_t *w, int32_t *h, int32_t *c,
+                        float *out, int64_t capacity);
+
 /* Fill `out` with pointers into the scene (valid until hpt_host_free):
  * the flattened tree of `KDTree::getData()` (kdtree.cpp:248-292) plus the
  * Scene vectors. */

@@ -8,46 +8,18 @@ resize of them."""
 from __future__ import annotations
 
 import os
-import shutil
 
 import numpy as np
 import pytest
 
 from conftest import GOLDEN
 from hptracing_amd import _native as N
-from hptracing_amd.scene import KDTree
+from hptracing_amd.scene import KDTree, read_image
+from textures import cimg_layer, load_textured
 
 PNG_DIR = os.path.join(GOLDEN, "png")
 EXPECTED = np.load(os.path.join(PNG_DIR, "expected.npz"))
 REFERENCE_SCENES = "/root/reference/scene"
-
-
-def cimg_layer(samples):
-    """normalize(0, 255) over every channel, then resize(512, 512, 1, 3)
-    with nearest neighbour (source index floor(dst * src / dst) on each
-    axis, channels included) and truncation to uint8 (CImg.h:19093-19105,
-    21063-21138)."""
-    f32 = np.float32
-    src = samples.astype(f32)
-    m, M = f32(src.min()), f32(src.max())
-    if m == M:
-        norm = np.zeros_like(src)
-    elif m != 0 or M != 255:
-        norm = ((src - m) / (M - m) * f32(255) + f32(0)).astype(f32)
-    else:
-        norm = src
-    h, w, c = src.shape
-    ys, xs = (np.arange(512) * h) // 512, (np.arange(512) * w) // 512
-    cs = (np.arange(3) * c) // 3
-    return norm[ys][:, xs][:, :, cs].astype(np.uint8)
-
-
-def load_textured(tmp_path, image_name, src_path):
-    shutil.copyfile(src_path, tmp_path / image_name)
-    (tmp_path / "m.mtl").write_text(f"newmtl tex\nKd 1 1 1\nmap_Kd {image_name}\n")
-    (tmp_path / "s.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\n"
-                                    "vt 0 0\nvt 1 0\nvt 0 1\nusemtl tex\nf 1/1 2/2 3/3\n")
-    return KDTree(str(tmp_path / "s.obj"), str(tmp_path) + "/")
 
 
 @pytest.mark.parametrize("name", sorted(EXPECTED.files))
@@ -57,6 +29,17 @@ def test_png_fixture_through_the_texture_pipeline(tmp_path, name):
     layer = kd.texture_data.reshape(512, 512, 4)
     assert np.array_equal(layer[:, :, :3], cimg_layer(EXPECTED[name]))
     assert (layer[:, :, 3] == 0).all()
+
+
+@pytest.mark.parametrize("name", sorted(EXPECTED.files))
+def test_png_fixture_samples(name):
+    """The whole decoded image, sample for sample (the 512x512 layer above
+    samples only part of a larger image)."""
+    got = read_image(os.path.join(PNG_DIR, name))
+    want = EXPECTED[name]
+    want = want[:, :, None] if want.ndim == 2 else want
+    assert got.shape == want.shape
+    assert np.array_equal(got, want.astype(np.float32))
 
 
 def _corrupt(tmp_path, name, edit):
@@ -70,7 +53,7 @@ def _corrupt(tmp_path, name, edit):
 @pytest.mark.parametrize("case,edit,message", [
     ("crc", lambda d: d.__setitem__(60, d[60] ^ 0x40), b"CRC"),
     ("truncated", lambda d: d.__delitem__(slice(len(d) // 2, None)), b"png"),
-    ("signature", lambda d: d.__setitem__(1, ord("Q")), b"only PNG and PNM"),
+    ("signature", lambda d: d.__setitem__(1, ord("Q")), b"only PNG, JPEG and PNM"),
 ])
 def test_png_errors(tmp_path, case, edit, message):
     src = _corrupt(tmp_path, "x.png", edit)
@@ -78,14 +61,6 @@ def test_png_errors(tmp_path, case, edit, message):
         load_textured(tmp_path, "x.png", src)
     assert e.value.code == N.HPT_ERR_IO
     assert message in str(e.value).encode()
-
-
-def test_jpeg_texture_is_a_clear_error(tmp_path):
-    p = tmp_path / "src.jpg"
-    p.write_bytes(b"\xff\xd8\xff\xe0\x00\x10JFIF\x00" + bytes(64))
-    with pytest.raises(N.HptError) as e:
-        load_textured(tmp_path, "t.jpg", p)
-    assert b"JPEG" in str(e.value).encode()
 
 
 @pytest.mark.skipif(not os.path.isdir(REFERENCE_SCENES), reason="reference scenes absent")
