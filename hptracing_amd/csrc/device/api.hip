@@ -173,17 +173,36 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         int64_t total = 0;
         upload(s->nodes, pnodes.data(), pnodes.size(), total);
         upload(s->tris, recs.data(), recs.size(), total);
-        // The same records once per triangle, with the leaf lists as
-        // triangle ids (the indirect fetch form): 48 bytes per triangle plus
-        // 4 per leaf entry instead of 48 per leaf entry.
+        // The same records once per triangle, with the leaf lists as record
+        // numbers (the indirect fetch form): 48 bytes per triangle plus 4 per
+        // leaf entry instead of 48 per leaf entry.  Records are numbered in
+        // the order the leaves first list their triangles (leaves in
+        // pre-order, entries in list order), so that a leaf's records, and
+        // those of the leaves next to it, lie together; each record keeps
+        // its triangle's index in its w slot, which is what a hit reports.
+        // The walk tests the entries in the same order either way.
         {
+            std::vector<int32_t> number(std::max<int64_t>(nt, 1), -1);
+            std::vector<int32_t> order;
+            order.reserve(nt);
+            for (int64_t e = 0; e < entries; ++e) {
+                int32_t &k = number[leaf_ids[e]];
+                if (k < 0) {
+                    k = (int32_t)order.size();
+                    order.push_back(leaf_ids[e]);
+                }
+                leaf_ids[e] = k;
+            }
+            for (int32_t tri = 0; tri < nt; ++tri)  // triangles no leaf lists
+                if (number[tri] < 0) order.push_back(tri);
             std::vector<float> trecs(12 * std::max<int64_t>(nt, 1), 0.f);
-            for (int32_t tri = 0; tri < nt; ++tri) {
+            for (int32_t k = 0; k < nt; ++k) {
+                const int32_t tri = order[k];
                 const int32_t *t = d->mesh + 4 * (int64_t)tri;
                 const float *pa = d->points + 4 * (int64_t)t[0];
                 const float *pb = d->points + 4 * (int64_t)t[1];
                 const float *pc = d->points + 4 * (int64_t)t[2];
-                float *r = &trecs[12 * (int64_t)tri];
+                float *r = &trecs[12 * (int64_t)k];
                 r[0] = pa[0]; r[1] = pa[1]; r[2] = pa[2];
                 std::memcpy(&r[3], &tri, 4);
                 r[4] = pb[0] - pa[0]; r[5] = pb[1] - pa[1]; r[6] = pb[2] - pa[2]; r[7] = 0.f;

@@ -218,8 +218,8 @@ struct LeafQueue {
 };
 
 // The record of leaf entry `at`: in leaf order (direct form), or through
-// the entry's triangle id in the per-triangle records (IND, indirect form);
-// both hold the same values.
+// the entry's record number in the per-triangle records (IND, indirect
+// form); both hold the same values, the triangle's index in w.
 template <bool IND>
 __device__ __forceinline__ const float4 *tri_record(const DevScene &sc, int at) {
     return IND ? sc.tri_recs + 3 * (int64_t)sc.leaf_ids[at] : sc.tris + 3 * (int64_t)at;
@@ -384,7 +384,7 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
             if (best < 0 || r < best) {
                 best = r;
                 const int at = queue_record<Q>(lq, threadIdx.x, nq, (int)(unsigned)kv);
-                best_id = IND ? sc.leaf_ids[at] : __float_as_int(sc.tris[3 * (int64_t)at].w);
+                best_id = __float_as_int(tri_record<IND>(sc, at)[0].w);
             }
         }
     }

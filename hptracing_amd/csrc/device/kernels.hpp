@@ -9,8 +9,9 @@
 //            b = offset of its first triangle record (count 0 = empty leaf).
 //   tris   : 48 bytes per leaf entry, in leaf-list order: float4 {pa, id},
 //            float4 {e1 = pb - pa, 0}, float4 {e2 = pc - pa, 0}.
-//   tri_recs, leaf_ids : the same records once per triangle, and the leaf
-//            lists as triangle ids (the indirect fetch form).
+//   tri_recs, leaf_ids : the same records once per triangle, numbered in
+//            the order the leaves first list them, and the leaf lists as
+//            those record numbers (the indirect fetch form).
 //   shading arrays keep the reference's layouts (float4 points/normals,
 //   float2 texcoords, int4 mesh and lights, 96-byte materials, RGBA8 texels).
 //
@@ -33,7 +34,7 @@ struct DevScene {
     int32_t n_nodes;
     const float4 *tris;
     const float4 *tri_recs;   // the records once per triangle (indirect form)
-    const int32_t *leaf_ids;  // triangle id per leaf entry (indirect form)
+    const int32_t *leaf_ids;  // record number per leaf entry (indirect form)
     const float4 *points;
     const float4 *normals;
     int32_t n_normals;
