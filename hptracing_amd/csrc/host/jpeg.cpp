@@ -470,6 +470,7 @@ Image decode(const std::vector<uint8_t> &file) {
             f.W = (s[3] << 8) | s[4];
             const int nf = s[5];
             if (f.H == 0 || f.W == 0) fail("image height given by DNL is not supported");
+            if ((int64_t)f.W * f.H > ((int64_t)1 << 28)) fail("image size out of range");
             if (nf == 4) fail("CMYK and YCCK images are not decoded");
             if (nf != 1 && nf != 3) fail("only 1- and 3-component images are decoded");
             if (sl < 6 + 3 * (size_t)nf) fail("bad frame header");
