@@ -230,6 +230,7 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.tris = s->tris.as<float4>();
         ds.tri_recs = s->tri_recs.as<float4>();
         ds.leaf_ids = s->leaf_ids.as<int32_t>();
+        ds.n_entries = (int32_t)std::max<int64_t>(entries, 1);
         ds.points = s->points.as<float4>();
         ds.normals = s->normals.as<float4>();
         ds.n_normals = (int32_t)d->n_normals;

@@ -60,28 +60,32 @@ __device__ __forceinline__ bool box_hit(V3 bs, V3 be, V3 sp, V3 d) {
 // answer is box_hit's for every input it is used on.
 //
 // Preconditions (checked by fast_box_ray per ray; the scene's box
-// coordinates are checked on upload): every direction component is 0 or
-// has magnitude in [2^-40, 2^40]; every origin and box coordinate is 0 or
-// has magnitude in [2^-60, 2^80].  Then every nonzero difference a = p - s
-// has |a| >= 2^-84 (both operands are multiples of 2^-84), so each quotient
-// is 0 or a normal number no larger than 2^121, and the approximate quotient
-// q' = fl(a * y) stays within relative 2^-21.9 of the correctly rounded
-// q = fl(a / d).  Zero, infinite and NaN quotients are identical in both
-// forms (a = 0, d = +-0: a * +-inf == a / +-0 with the same sign rules).
+// coordinates are checked on upload): every direction component has
+// magnitude in [2^-40, 2^40] (none is 0); every origin and box coordinate
+// is 0 or has magnitude in [2^-60, 2^80].  Then every nonzero difference
+// a = p - s has |a| >= 2^-84 (both operands are multiples of 2^-84), so
+// each quotient is 0 or a normal number no larger than 2^121 (never
+// infinite or NaN), and the approximate quotient q' = fl(a * y) stays
+// within relative 2^-21.9 of the correctly rounded q = fl(a / d), with the
+// same sign class (negative, zero, positive).
 //
-//  * "nearest exit <= 0" depends only on the sign class (negative, zero,
-//    positive, NaN) of each quotient, which q' shares with q: exact.
-//  * fminf/fmaxf return one of their inputs, and x -> x -+ e|x| is monotone
-//    for e < 1, so the entry (max of mins) and exit (min of maxs) formed
-//    from q' lie within relative e of those formed from q.  When they differ
-//    by more than 2^-16 of their magnitudes (far above 2 * 2^-21.9 plus the
-//    rounding of the check itself), or when either is infinite or NaN (then
-//    exact), their order is the exact order.  Otherwise the division form
-//    decides.
+//  * The exit (min of maxs) formed from q' has the sign class of the one
+//    formed from q: fminf/fmaxf pick one of their inputs, and the inputs'
+//    sign classes agree one for one.  So "nearest exit <= 0" is exact.
+//  * A start point inside the closed box has lo - p <= 0 <= hi - p on every
+//    axis, so each axis's quotients straddle 0 and entry <= 0 <= exit.  With
+//    exit > 0 the box is then hit by the slab rule as well, and exit < 0
+//    rules the start point out; only exit == 0 needs the start-inside test,
+//    and that rare case is left to the division form.
+//  * x -> x -+ e|x| is monotone for e < 1, so the entry (max of mins) and
+//    exit formed from q' lie within relative e of those formed from q.
+//    When they differ by more than 2^-16 of their magnitudes (far above
+//    2 * 2^-21.9 plus the rounding of the check itself), their order is the
+//    exact order.  Otherwise the division form decides.
 __device__ __forceinline__ bool fast_box_ray(V3 sp, V3 d) {
     auto dir_ok = [](float c) {
         const float a = fabsf(c);
-        return a == 0.f || (a >= 0x1p-40f && a <= 0x1p40f);
+        return a >= 0x1p-40f && a <= 0x1p40f;
     };
     auto org_ok = [](float c) {
         const float a = fabsf(c);
@@ -96,41 +100,25 @@ __device__ __forceinline__ V3 approx_rcp(V3 d) {
 }
 
 //
-// The start-inside test reads the same differences: for finite floats (the
-// preconditions) p - s <= 0 exactly when s >= p, with gradual underflow.
 // The certified part alone: the reciprocal form's decision, and in `open`
 // whether it is not certified (the caller then asks box_hit).
 __device__ __forceinline__ bool box_hit_rcp_part(V3 bs, V3 be, V3 sp, V3 y, bool &open) {
-    const float ax0 = bs.x - sp.x, ax1 = be.x - sp.x;
-    const float ay0 = bs.y - sp.y, ay1 = be.y - sp.y;
-    const float az0 = bs.z - sp.z, az1 = be.z - sp.z;
-    const bool inside = fmaxf(fmaxf(ax0, ay0), az0) <= 0.f && fminf(fminf(ax1, ay1), az1) >= 0.f;
-    const float tx0 = ax0 * y.x, tx1 = ax1 * y.x;
-    const float ty0 = ay0 * y.y, ty1 = ay1 * y.y;
-    const float tz0 = az0 * y.z, tz1 = az1 * y.z;
+    const float tx0 = (bs.x - sp.x) * y.x, tx1 = (be.x - sp.x) * y.x;
+    const float ty0 = (bs.y - sp.y) * y.y, ty1 = (be.y - sp.y) * y.y;
+    const float tz0 = (bs.z - sp.z) * y.z, tz1 = (be.z - sp.z) * y.z;
     const float entry = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     const float exit = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
     const float bound = (fabsf(entry) + fabsf(exit)) * 0x1p-16f;
-    const bool sure = fabsf(entry - exit) >= bound || !(bound < INFINITY);
+    const bool sure = fabsf(entry - exit) >= bound;
     const bool ahead = exit > 0.f;
-    open = !inside && ahead && !sure;
-    return inside || (ahead && entry <= exit);
+    open = exit == 0.f || (ahead && !sure);
+    return ahead && entry <= exit;
 }
 
 __device__ __forceinline__ bool box_hit_rcp(V3 bs, V3 be, V3 sp, V3 d, V3 y) {
-    const float ax0 = bs.x - sp.x, ax1 = be.x - sp.x;
-    const float ay0 = bs.y - sp.y, ay1 = be.y - sp.y;
-    const float az0 = bs.z - sp.z, az1 = be.z - sp.z;
-    const bool inside = fmaxf(fmaxf(ax0, ay0), az0) <= 0.f && fminf(fminf(ax1, ay1), az1) >= 0.f;
-    const float tx0 = ax0 * y.x, tx1 = ax1 * y.x;
-    const float ty0 = ay0 * y.y, ty1 = ay1 * y.y;
-    const float tz0 = az0 * y.z, tz1 = az1 * y.z;
-    const float entry = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
-    const float exit = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-    const float bound = (fabsf(entry) + fabsf(exit)) * 0x1p-16f;
-    const bool sure = fabsf(entry - exit) >= bound || !(bound < INFINITY);
-    const bool ahead = exit > 0.f;
-    if (inside || !ahead || sure) return inside || (ahead && entry <= exit);
+    bool open;
+    const bool hit = box_hit_rcp_part(bs, be, sp, y, open);
+    if (!open) return hit;
     return box_hit(bs, be, sp, d);
 }
 

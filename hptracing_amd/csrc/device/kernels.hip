@@ -180,41 +180,46 @@ __device__ __forceinline__ int wave_scan_max(int v, int lane) {
     return wave_scan(v, lane, [](int a, int b) { return a > b ? a : b; });
 }
 
-// Segmented fold of (r, id) pairs in lane order: a segment starts at a lane
-// with `seg` set.  The operator keeps the earlier entry unless only the
-// later has r > 0 or the later's r is smaller -- the reference's
+// Segmented fold of (r, id) pairs in lane order.  A lane's segment is the
+// run of lanes from `seg_start` up to it (positions of one owner are
+// contiguous).  The operator keeps the earlier entry unless only the later
+// has r > 0 or the later's r is smaller -- the reference's
 // `r > 0 && (best < 0 || r < best)` applied in order -- and is associative,
-// so after the scan each lane holds the sequential fold of its segment up to
-// itself.
+// so the scan (steps 1, 2, 4, 8 inside each 16-lane row, then the last lane
+// of rows 0 and 2 into rows 1 and 3, then lane 31 into the upper half)
+// leaves each lane with the sequential fold from its segment's start to
+// itself: a lane takes a step's source only where the source lies in its
+// segment, and the source's range then ends where the lane's begins.
 template <int CTRL>
-__device__ __forceinline__ void seg_step(int step, int lane, float &r, int &id, bool &seg) {
+__device__ __forceinline__ void fold_step(bool take, float &r, int &id) {
     const float r_up = __int_as_float(dpp_mov<CTRL>(__float_as_int(r)));
     const int id_up = dpp_mov<CTRL>(id);
-    const bool seg_up = dpp_mov<CTRL>(seg ? 1 : 0) != 0;
-    if (dpp_step_valid(step, lane) && !seg) {
-        if (r_up > 0 && !(r > 0 && r < r_up)) {
-            r = r_up;
-            id = id_up;
-        }
-        seg = seg_up;
+    if (take && r_up > 0 && !(r > 0 && r < r_up)) {
+        r = r_up;
+        id = id_up;
     }
 }
-__device__ __forceinline__ void wave_seg_fold(int lane, float &r, int &id, bool seg) {
-    seg_step<0x111>(1, lane, r, id, seg);
-    seg_step<0x112>(2, lane, r, id, seg);
-    seg_step<0x114>(4, lane, r, id, seg);
-    seg_step<0x118>(8, lane, r, id, seg);
-    seg_step<0x142>(16, lane, r, id, seg);
-    seg_step<0x143>(32, lane, r, id, seg);
+__device__ __forceinline__ void wave_seg_fold(int lane, int seg_start, float &r, int &id) {
+    const int rel = lane - seg_start;  // lanes of the segment before this one
+    fold_step<0x111>((lane & 15) >= 1 && rel >= 1, r, id);
+    fold_step<0x112>((lane & 15) >= 2 && rel >= 2, r, id);
+    fold_step<0x114>((lane & 15) >= 4 && rel >= 4, r, id);
+    fold_step<0x118>((lane & 15) >= 8 && rel >= 8, r, id);
+    fold_step<0x142>((lane & 31) >= 16 && (lane & ~15) - 1 >= seg_start, r, id);
+    fold_step<0x143>(lane >= 32 && 31 >= seg_start, r, id);
 }
 
-// Per-lane queue of pending leaves in LDS (FLAT form): up to Q entries of
-// (first record, triangle count) per lane, lane-major so that a wave's
-// accesses to one slot fall in distinct banks.
+// Per-lane queue of pending leaves in LDS (FLAT form): up to Q entries per
+// lane, lane-major so that a wave's accesses to one slot fall in distinct
+// banks.  A lane's work is the triangles of its queued leaves in queue
+// order; for the leaf in slot q, start[q] is the position of its first
+// triangle in that work (the counts of the leaves before it) and base[q]
+// its first leaf entry minus start[q], so that position k of the lane's
+// work is leaf entry base[q] + k for the last q with start[q] <= k.
 template <int Q>
 struct LeafQueue {
-    int off[Q + 1][kBlock];  // slot Q takes the writes of lanes that queue nothing
-    int cnt[Q + 1][kBlock];
+    int base[Q + 1][kBlock];   // slot Q takes the writes of lanes that queue nothing
+    int start[Q + 1][kBlock];
 };
 
 // The record of leaf entry `at`: in leaf order (direct form), or through
@@ -225,11 +230,27 @@ __device__ __forceinline__ const float4 *tri_record(const DevScene &sc, int at) 
     return IND ? sc.tri_recs + 3 * (int64_t)sc.leaf_ids[at] : sc.tris + 3 * (int64_t)at;
 }
 
+// Leaf entry of position k (0 <= k < the lane's count) of the work of the
+// lane whose queue is in column `col`, with `nq` leaves queued: the slots
+// from nq on hold stale or never-written values and do not take part.  The
+// starts of the queued leaves increase (every queued leaf has triangles),
+// so the leaf holding k is the number of later slots starting at or before
+// k.  The entry is clamped into the scene's leaf lists, so that no error in
+// the bookkeeping can read outside them.
+template <int Q>
+__device__ __forceinline__ int queue_record(const LeafQueue<Q> &lq, int col, int nq, int k,
+                                            int n_entries) {
+    int q = 0;
+#pragma unroll
+    for (int s = 1; s < Q; ++s) q += (s < nq && lq.start[s][col] <= k) ? 1 : 0;
+    const int at = lq.base[q][col] + k;
+    return min(max(at, 0), n_entries - 1);
+}
+
 // One leaf round of the FLAT form (see cast_refill).  A lane's pending work
 // is the triangles of its `nq` queued leaves in queue order, `cnt` in all
-// (0 and 0 if none); best / best_id are folded in place.  Queue slots from
-// nq on hold stale or never-written values and are not read.  All 64 lanes
-// must call it.
+// (0 and 0 if none); best / best_id are folded in place.  All 64 lanes must
+// call it.
 template <int Q, bool IND>
 __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
                                                 int lane, int nq, int cnt, V3 sp, V3 d,
@@ -259,22 +280,9 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
         float r = -1.f;
         int id = -1;
         if (j < total) {
-            // Position k of the owner's work: record k of its first queued
-            // leaf, or further along its queue (0 <= k < the owner's count,
-            // the sum of the counts of its o_nq queued leaves).
-            // Clamped into the selected leaf, so that a record outside the
-            // owner's leaves is never read.
-            int k = j - o_excl;
-            int c = lq.cnt[0][w + own];
-            int at = lq.off[0][w + own] + min(max(k, 0), c - 1);
-#pragma unroll
-            for (int q = 1; q < Q; ++q) {
-                if (q < o_nq) {
-                    k -= c;
-                    c = lq.cnt[q][w + own];
-                    if (k >= 0) at = lq.off[q][w + own] + min(k, c - 1);
-                }
-            }
+            // Position j - o_excl of the owner's work (0 <= it < the owner's
+            // count, the sum of the counts of its o_nq queued leaves).
+            const int at = queue_record<Q>(lq, w + own, o_nq, j - o_excl, sc.n_entries);
             const float4 *rec = tri_record<IND>(sc, at);
             const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
             r = sc.arith_ocl ? tri_hit_ocl(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2))
@@ -282,12 +290,10 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
             id = __float_as_int(r0.w);
         }
         // Fold the chunk per owner: a segment is one owner's run of
-        // positions, and the chunk start also starts one.  (Every lane
-        // executes each cross-lane move: a lane that skipped one would
-        // supply no value to its neighbour.)
-        const int own_prev = __shfl(own, lane > 0 ? lane - 1 : 0, 64);
-        const bool seg = lane == 0 || own != own_prev;
-        wave_seg_fold(lane, r, id, seg);
+        // positions, starting at its first position or at the chunk start.
+        // (Every lane executes each cross-lane move: a lane that skipped one
+        // would supply no value to its neighbour.)
+        wave_seg_fold(lane, max(o_excl - base, 0), r, id);
         // Each owner with positions in this chunk takes its segment's fold,
         // which follows everything it has folded before.
         const int last = (excl + cnt < base + 64 ? excl + cnt : base + 64) - 1 - base;
@@ -309,25 +315,6 @@ struct RaySlots {
     float4 org[kBlock];
     float4 dir[kBlock];
 };
-
-// Record index of position k (0 <= k < the owner's count) of the work of the
-// lane whose queue is in column `col`: record k of its first queued leaf, or
-// further along its queue.  Clamped into the selected leaf, so that a record
-// outside the owner's leaves is never read.
-template <int Q>
-__device__ __forceinline__ int queue_record(const LeafQueue<Q> &lq, int col, int nq, int k) {
-    int c = lq.cnt[0][col];
-    int at = lq.off[0][col] + min(max(k, 0), c - 1);
-#pragma unroll
-    for (int q = 1; q < Q; ++q) {
-        if (q < nq) {
-            k -= c;
-            c = lq.cnt[q][col];
-            if (k >= 0) at = lq.off[q][col] + min(k, c - 1);
-        }
-    }
-    return at;
-}
 
 // The leaf round with a keyed fold (KEYED form).  The reference keeps, of a
 // ray's tests in order, the first with the smallest r > 0.  For r > 0
@@ -364,7 +351,7 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
         const int j = base + lane;
         if (j < total) {
             const int k = j - o_excl;
-            const int at = queue_record<Q>(lq, w + own, o_nq, k);
+            const int at = queue_record<Q>(lq, w + own, o_nq, k, sc.n_entries);
             const float4 o4 = rs.org[w + own], d4 = rs.dir[w + own];
             const float4 *rec = tri_record<IND>(sc, at);
             const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
@@ -383,7 +370,8 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
             const float r = __uint_as_float((unsigned)(kv >> 32));
             if (best < 0 || r < best) {
                 best = r;
-                const int at = queue_record<Q>(lq, threadIdx.x, nq, (int)(unsigned)kv);
+                const int at = queue_record<Q>(lq, threadIdx.x, nq, (int)(unsigned)kv,
+                                               sc.n_entries);
                 best_id = __float_as_int(tri_record<IND>(sc, at)[0].w);
             }
         }
@@ -443,6 +431,11 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
     int64_t ray = -1;
     V3 sp = mk(0.f, 0.f, 0.f), d = sp, inv = sp;
     bool rcp = false, done = true;
+    // Node q's two halves (box_start + a, box_end + b).
+    auto fetch = [&](int q, float4 &lo, float4 &hi) {
+        lo = sc.nodes[2 * q];
+        hi = sc.nodes[2 * q + 1];
+    };
     // pending: leaves held for the next leaf round (FLAT: queued in LDS,
     // pcnt = their triangle total; otherwise pcnt / poff of the one leaf).
     int idx = 0, leaves = 0, best_id = -1, pcnt = 0, poff = 0, pending = 0;
@@ -462,8 +455,8 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
         } else {
             if (hit && a > 0) {
                 if (FLAT) {
-                    lq.off[pending][tid] = b;
-                    lq.cnt[pending][tid] = a;
+                    lq.base[pending][tid] = b - pcnt;
+                    lq.start[pending][tid] = pcnt;
                     pcnt += a;
                 } else {
                     pcnt = a;
@@ -493,8 +486,8 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
         const bool take = leaf && hit && a > 0;
         idx = (!leaf && !hit) ? a : idx + 1;
         const int slot = take ? pending : QN;
-        lq.off[slot][tid] = b;
-        lq.cnt[slot][tid] = a;
+        lq.base[slot][tid] = b - pcnt;
+        lq.start[slot][tid] = pcnt;
         pcnt += take ? a : 0;
         pending += take ? 1 : 0;
         done = done || (take && leaves + pending == kMaxMatchData) || idx >= nn;
@@ -559,8 +552,8 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                 const uint64_t wm = __ballot(walk);
                 if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
                 if (walk) {
-                    const float4 lo = sc.nodes[2 * idx];
-                    const float4 hi = sc.nodes[2 * idx + 1];
+                    float4 lo, hi;
+                    fetch(idx, lo, hi);
                     visit_rcp(lo, hi);
                 }
             }
@@ -576,19 +569,16 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                         // loads past the last node repeat that node.
                         float4 lo[WIN], hi[WIN];
 #pragma unroll
-                        for (int j = 0; j < WIN; ++j) {
-                            const int q = idx + j < nn ? idx + j : nn - 1;
-                            lo[j] = sc.nodes[2 * q];
-                            hi[j] = sc.nodes[2 * q + 1];
-                        }
+                        for (int j = 0; j < WIN; ++j)
+                            fetch(idx + j < nn ? idx + j : nn - 1, lo[j], hi[j]);
                         const int first = idx;
                         visit(lo[0], hi[0]);
 #pragma unroll
                         for (int j = 1; j < WIN; ++j)
                             if (!done && pending < QN && idx == first + j) visit(lo[j], hi[j]);
                     } else {
-                        const float4 lo = sc.nodes[2 * idx];
-                        const float4 hi = sc.nodes[2 * idx + 1];
+                        float4 lo, hi;
+                        fetch(idx, lo, hi);
                         visit(lo, hi);
                     }
                 }

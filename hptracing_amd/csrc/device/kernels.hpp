@@ -35,6 +35,7 @@ struct DevScene {
     const float4 *tris;
     const float4 *tri_recs;   // the records once per triangle (indirect form)
     const int32_t *leaf_ids;  // record number per leaf entry (indirect form)
+    int32_t n_entries;        // leaf entries (length of tris and leaf_ids; at least 1)
     const float4 *points;
     const float4 *normals;
     int32_t n_normals;

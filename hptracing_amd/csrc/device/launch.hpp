@@ -41,7 +41,8 @@ int occupancy_blocks_shade();
 // and (without node pairs) bit 6 (64) folds the leaf rounds through keys in
 // LDS; bit 7 (128)
 // reads the triangle records once per triangle through the leaf entries'
-// ids.  These are the compiled forms.
+// ids; bits 8-9 (with 1) widen the node window.  These are the compiled
+// forms.
 inline bool valid_fetch(int f) {
     switch (f) {
     case 0: case 1: case 2: case 3: case 4: case 5: case 12: case 13: case 20: case 21:

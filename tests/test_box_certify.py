@@ -2,8 +2,9 @@
 hptracing_amd/csrc/device/dmath.hpp) as a method, on the CPU: quotients
 formed as (p - s) * y with y within 1 ulp of 1/d (here the correctly rounded
 reciprocal moved by -1, 0 or +1 ulp at random, the accuracy of v_rcp_f32),
-entry/exit compared with the 2^-16 relative margin, and the division form
-where the margin does not certify.  Every decision must equal the division
+entry/exit compared with the 2^-16 relative margin, no start-inside test, and
+the division form where the margin does not certify, where the exit is 0, and
+for rays with a zero direction component.  Every decision must equal the division
 form's (the slab test of kernel.cl:51-83, restated by the oracle) on
 adversarial rays aimed at box corners, edges and faces, and the fallback
 must be rare on random rays.  The GPU kernels themselves are checked against the oracle by
@@ -31,13 +32,17 @@ def box_div(bs, be, sp, d):
 
 
 def box_rcp(bs, be, sp, d, y):
+    """The certified form: no start-inside test (a start point inside the
+    box makes entry <= 0 <= exit, so only exit == 0 needs it), and rays with
+    a zero direction component take the division form throughout, as
+    fast_box_ray sends them there."""
     with np.errstate(all="ignore"):
-        inside, entry, exit_ = _slab(bs, be, sp, (bs - sp) * y, (be - sp) * y)
+        _, entry, exit_ = _slab(bs, be, sp, (bs - sp) * y, (be - sp) * y)
         bound = (np.abs(entry) + np.abs(exit_)) * F(2.0 ** -16)
-        sure = (np.abs(entry - exit_) >= bound) | ~(bound < np.inf)
+        sure = np.abs(entry - exit_) >= bound
     ahead = exit_ > 0
-    fast = inside | (ahead & (entry <= exit_))
-    need = ~inside & ahead & ~sure
+    fast = ahead & (entry <= exit_)
+    need = (exit_ == 0) | (ahead & ~sure) | np.any(d == 0, axis=1)
     out = fast.copy()
     out[need] = box_div(bs[need], be[need], sp[need], d[need])
     return out, need
@@ -117,3 +122,30 @@ def test_margin_is_needed():
         inside, entry, exit_ = _slab(lo, hi, org, (lo - org) * y, (hi - org) * y)
     naive = inside | ((exit_ > 0) & (entry <= exit_))
     assert (naive != box_div(lo, hi, org, d)).sum() > 0
+
+
+def test_start_inside_without_inside_test():
+    """Start points inside the closed box, on its faces, edges and corners,
+    with directions that have no zero component: the certified form, which
+    has no start-inside test, decides as the division form does; and the
+    exit == 0 fallback is what keeps it exact (without it, start points on
+    a face the ray leaves through are reported as misses)."""
+    rng = np.random.default_rng(10)
+    n = 400_000
+    lo = rng.uniform(-10, 10, (n, 3)).astype(F)
+    hi = (lo + rng.uniform(0.001, 5, (n, 3))).astype(F)
+    u = rng.uniform(0, 1, (n, 3)).astype(F)
+    snap = rng.integers(0, 3, (n, 3))          # 0: inside, 1: on lo, 2: on hi
+    org = np.where(snap == 1, lo, np.where(snap == 2, hi, (lo + u * (hi - lo)).astype(F)))
+    org = np.clip(org, lo, hi).astype(F)
+    d = rng.normal(size=(n, 3)).astype(F)
+    d[d == 0] = F(1)
+    y = _approx_rcp(d, rng)
+    ref = box_div(lo, hi, org, d)
+    assert ref.all()                           # every start point is in the closed box
+    got, need = box_rcp(lo, hi, org, d, y)
+    assert np.array_equal(got, ref)
+    with np.errstate(all="ignore"):
+        _, entry, exit_ = _slab(lo, hi, org, (lo - org) * y, (hi - org) * y)
+    no_fallback = (exit_ > 0) & (entry <= exit_)
+    assert (no_fallback != ref).sum() > 0

@@ -33,18 +33,15 @@ def owners_by_chunk(cnt):
     return out, excl
 
 
-def queue_record(off, cnt_q, nq, k):
-    """Record of position k in a queue of nq leaves (offsets off, counts
-    cnt_q), clamped into the selected leaf (queue_record in kernels.hip)."""
-    c = cnt_q[0]
-    at = off[0] + min(max(k, 0), c - 1)
-    for q in range(1, len(off)):
-        if q < nq:
-            k -= c
-            c = cnt_q[q]
-            if k >= 0:
-                at = off[q] + min(k, c - 1)
-    return at
+def queue_record(base, start, nq, k, n_entries):
+    """Leaf entry of position k of a lane's work (queue_record in
+    kernels.hip): slot q holds start[q], the position of its leaf's first
+    triangle, and base[q], its first entry minus start[q]; slots from nq on
+    hold stale values and do not take part.  The leaf holding k is the
+    number of later queued slots starting at or before k; the entry is
+    clamped into the scene's leaf lists."""
+    q = sum(1 for s in range(1, len(base)) if s < nq and start[s] <= k)
+    return min(max(int(base[q]) + k, 0), n_entries - 1)
 
 
 def seq_fold(best, best_id, rs, ids):
@@ -67,23 +64,47 @@ def keyed_fold(best, best_id, rs, ids):
 
 def test_positions_map_to_their_owner_and_record():
     rng = np.random.default_rng(1)
-    Q = 4
-    for trial in range(300):
-        nq = rng.integers(0, Q + 1, WAVE)
-        nq[rng.random(WAVE) < 0.3] = 0
-        cnt_q = rng.integers(1, 40, (WAVE, Q))
-        off = rng.integers(0, 10**6, (WAVE, Q))
-        cnt = np.array([cnt_q[l, :nq[l]].sum() for l in range(WAVE)])
-        if cnt.sum() == 0:
-            continue
-        own, excl = owners_by_chunk(cnt)
-        for j, o in enumerate(own):
-            k = j - excl[o]
-            assert 0 <= k < cnt[o], (trial, j)
-            at = queue_record(off[o], cnt_q[o], nq[o], k)
-            # the k-th record of the owner's queued leaves, in queue order
-            flat = np.concatenate([off[o, q] + np.arange(cnt_q[o, q]) for q in range(nq[o])])
-            assert at == flat[k]
+    n_entries = 10**6 + 64
+    for Q in (4, 8):
+        for trial in range(300):
+            nq = rng.integers(0, Q + 1, WAVE)
+            nq[rng.random(WAVE) < 0.3] = 0
+            cnt_q = rng.integers(1, 40, (WAVE, Q))
+            off = rng.integers(0, 10**6, (WAVE, Q))
+            # what the lanes wrote when queueing; the slots from nq on keep
+            # whatever an earlier round left there
+            start = rng.integers(-50, 400, (WAVE, Q))
+            base = rng.integers(-10**6, 2 * 10**6, (WAVE, Q))
+            for l in range(WAVE):
+                s0 = 0
+                for q in range(nq[l]):
+                    start[l, q] = s0
+                    base[l, q] = off[l, q] - s0
+                    s0 += cnt_q[l, q]
+            cnt = np.array([cnt_q[l, :nq[l]].sum() for l in range(WAVE)])
+            if cnt.sum() == 0:
+                continue
+            own, excl = owners_by_chunk(cnt)
+            for j, o in enumerate(own):
+                k = j - excl[o]
+                assert 0 <= k < cnt[o], (trial, j)
+                at = queue_record(base[o], start[o], nq[o], k, n_entries)
+                # the k-th record of the owner's queued leaves, in queue order
+                flat = np.concatenate([off[o, q] + np.arange(cnt_q[o, q]) for q in range(nq[o])])
+                assert at == flat[k]
+
+
+def test_queue_record_stays_in_the_leaf_lists():
+    """Whatever the queue holds, the entry read is inside the leaf lists."""
+    rng = np.random.default_rng(4)
+    for _ in range(20000):
+        Q = 8
+        base = rng.integers(-2**31, 2**31 - 2**20, Q)
+        start = rng.integers(-2**20, 2**20, Q)
+        nq = int(rng.integers(0, Q + 1))
+        k = int(rng.integers(0, 2**20))
+        n_entries = int(rng.integers(1, 2**30))
+        assert 0 <= queue_record(base, start, nq, k, n_entries) < n_entries
 
 
 def test_keyed_fold_equals_sequential_fold():
@@ -108,3 +129,55 @@ def test_positive_float_bits_order_as_values():
     x = x[x > 0]
     assert np.array_equal(np.argsort(x, kind="stable"),
                           np.argsort(x.view(np.uint32), kind="stable"))
+
+
+def dpp_fold(r, ids, seg_start):
+    """The leaf round's segmented fold (wave_seg_fold in kernels.hip) lane
+    by lane: steps 1, 2, 4, 8 take lane - step inside each 16-lane row,
+    then rows 1 and 3 take the last lane of the row before, then lanes
+    32..63 take lane 31; a lane takes a source only if the source lies in
+    its segment (at or after seg_start).  Every lane reads the values the
+    step started with."""
+    r = np.array(r, np.float32)
+    ids = np.array(ids)
+    lanes = np.arange(WAVE)
+
+    def step(src, take):
+        nr, ni = r.copy(), ids.copy()
+        for l in lanes:
+            if take[l]:
+                ru, iu = r[src[l]], ids[src[l]]
+                if ru > 0 and not (r[l] > 0 and r[l] < ru):
+                    nr[l], ni[l] = ru, iu
+        return nr, ni
+
+    rel = lanes - seg_start
+    for s in (1, 2, 4, 8):
+        r, ids = step(np.maximum(lanes - s, 0), ((lanes & 15) >= s) & (rel >= s))
+    src16 = (lanes & ~15) - 1
+    r, ids = step(np.maximum(src16, 0), ((lanes & 31) >= 16) & (src16 >= seg_start))
+    r, ids = step(np.full(WAVE, 31), (lanes >= 32) & (31 >= seg_start))
+    return r, ids
+
+
+def test_segmented_fold_is_the_sequential_fold_per_segment():
+    """Each lane ends with the sequential fold of its segment from the
+    segment's start to itself, for random segmentations of a chunk (owners'
+    runs of positions), with misses, ties and infinities."""
+    rng = np.random.default_rng(5)
+    pool = np.array([-1.0, 0.5, 0.25, 0.5, np.inf, 3.0, 2.0], np.float32)
+    for trial in range(400):
+        cuts = np.sort(rng.choice(np.arange(1, WAVE), int(rng.integers(0, 20)), replace=False))
+        starts = np.concatenate([[0], cuts])
+        seg_start = np.zeros(WAVE, np.int64)
+        for a in starts:
+            seg_start[a:] = a
+        r = rng.choice(pool, WAVE) if trial % 2 else rng.uniform(-1, 4, WAVE).astype(np.float32)
+        ids = rng.integers(0, 1000, WAVE)
+        fr, fi = dpp_fold(r, ids, seg_start)
+        for l in range(WAVE):
+            b, bi = seq_fold(np.float32(-42.0), -1, r[seg_start[l]:l + 1], ids[seg_start[l]:l + 1])
+            if bi == -1:     # nothing positive: the lane keeps its own entry
+                assert not (fr[l] > 0), (trial, l)
+                continue
+            assert fr[l] == b and fi[l] == bi, (trial, l)
