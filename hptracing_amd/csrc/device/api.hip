@@ -295,15 +295,16 @@ int hpt_scene_stats_get(const hpt_scene *s, hpt_scene_stats *out) {
 int hpt_scene_get_tuning(const hpt_scene *s, hpt_tuning *out) {
     return guarded([&] {
         if (!s || !out) fail(HPT_ERR_INVALID, "null argument");
-        out->fast_division = s->dev.fast_div;
-        out->cast_blocks_per_cu = s->cast_grid / std::max(1, s->n_cus);
-        out->shade_blocks_per_cu = s->shade_grid / std::max(1, s->n_cus);
-        out->traversal = s->dev.traversal;
-        out->refill_threshold = s->dev.refill;
-        out->node_steps = s->dev.node_steps;
-        out->fetch = s->dev.fetch;
-        out->walk_min = s->dev.walk_min;
-        out->sort_rays = s->sort_rays;
+        const hpt::LaunchForm f = s->form();
+        out->fast_division = f.dev.fast_div;
+        out->cast_blocks_per_cu = f.cast_grid / std::max(1, s->n_cus);
+        out->shade_blocks_per_cu = f.shade_grid / std::max(1, s->n_cus);
+        out->traversal = f.dev.traversal;
+        out->refill_threshold = f.dev.refill;
+        out->node_steps = f.dev.node_steps;
+        out->fetch = f.dev.fetch;
+        out->walk_min = f.dev.walk_min;
+        out->sort_rays = f.sort_rays;
     });
 }
 
@@ -318,6 +319,7 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
             t->sort_rays < 0 || t->sort_rays > 1)
             fail(HPT_ERR_INVALID, "tuning value out of range");
         std::lock_guard<std::mutex> lock(s->mu);
+        std::lock_guard<std::mutex> tlock(s->tune_mu);
         s->dev.fast_div = t->fast_division ? 1 : 0;
         s->dev.traversal = t->traversal;
         s->dev.refill = t->refill_threshold;
@@ -336,6 +338,7 @@ int hpt_scene_set_arithmetic(hpt_scene *s, int mode) {
         if (mode != HPT_ARITH_LITERAL && mode != HPT_ARITH_ROCM_OPENCL)
             fail(HPT_ERR_INVALID, "unknown arithmetic mode");
         std::lock_guard<std::mutex> lock(s->mu);
+        std::lock_guard<std::mutex> tlock(s->tune_mu);
         s->dev.arith_ocl = mode == HPT_ARITH_ROCM_OPENCL ? 1 : 0;
     });
 }
@@ -352,13 +355,14 @@ static int cast_impl(hpt_scene *s, const float *org, const float *dir, int64_t n
         // NULL is HIP's null stream, so a cast follows whatever the caller
         // enqueued there to produce its rays.
         hipStream_t st = (hipStream_t)stream;
-        if (!counts && s->sort_rays && n >= hpt::kSortMinRays) {
-            hpt::cast_sorted(s, org, dir, n, tri_id, t, st);
+        const hpt::LaunchForm f = s->form();
+        if (!counts && f.sort_rays && n >= hpt::kSortMinRays) {
+            hpt::cast_sorted(s, f, org, dir, n, tri_id, t, st);
             return;
         }
         hpt::StreamCounter wc(st);
-        hpt::launch_cast(s->dev, org, dir, nullptr, nullptr, hpt::CountRef{}, n, wc.p, tri_id, t,
-                         counts, nullptr, s->cast_grid, st);
+        hpt::launch_cast(f.dev, org, dir, nullptr, nullptr, hpt::CountRef{}, n, wc.p, tri_id, t,
+                         counts, nullptr, f.cast_grid, st);
         hip_check(hipGetLastError(), "k_cast launch");
     });
 }

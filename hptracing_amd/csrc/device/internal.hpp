@@ -205,8 +205,22 @@ struct KernelTimer {
     }
 };
 
+// The launch settings of a scene, copied out under its settings lock by
+// callers that launch without holding the scene's render lock.
+struct LaunchForm {
+    DevScene dev;
+    int cast_grid, shade_grid, sort_rays;
+};
+
 }  // namespace hpt
 
+// Locking: `mu` is held by renders, batch renders and the render autotuner
+// for their whole run (they share `ws` and `stream`).  The launch settings
+// (dev, cast_grid, shade_grid, sort_rays) change only under `mu` and
+// `tune_mu` together, so a holder of `mu` reads them directly, and every
+// other caller (hpt_cast, hpt_cast_noalloc, the server's worker, the cast
+// autotuner) takes a copy with form(), which holds only `tune_mu` and so
+// never waits for a render.
 struct hpt_scene {
     int device = 0;
     hpt::DevScene dev{};
@@ -218,8 +232,13 @@ struct hpt_scene {
     int sort_rays = 0;                 // hpt_tuning.sort_rays
     float box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // root node's box
     std::mutex mu;
+    mutable std::mutex tune_mu;
     hpt::Workspace ws;  // hpt_render / hpt_batch_render / hpt_autotune, under mu
     hipStream_t stream = nullptr;
+    hpt::LaunchForm form() const {
+        std::lock_guard<std::mutex> lock(tune_mu);
+        return hpt::LaunchForm{dev, cast_grid, shade_grid, sort_rays};
+    }
 };
 
 namespace hpt {
@@ -228,8 +247,8 @@ namespace hpt {
 constexpr int64_t kSortMinRays = 65536;
 // hpt_cast of a batch in Morton order of its origins (sort.hip); results
 // in the caller's order.  Buffers from the stream-ordered allocator.
-void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
-                 float *out_t, hipStream_t st);
+void cast_sorted(const hpt_scene *s, const LaunchForm &f, const float *org3, const float *dir3,
+                 int64_t n, int *out_id, float *out_t, hipStream_t st);
 // The same with the caller's buffers, each of n entries (keys and idx
 // twice: the radix sort's double buffer), temp_bytes >= sort_temp_bytes(n)
 // and a work counter cleared to zero.
@@ -244,6 +263,7 @@ struct SortBuffers {
     WorkCounter *wc;
 };
 size_t sort_temp_bytes(int64_t n, hipStream_t st);
-void cast_sorted_into(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
-                      float *out_t, const SortBuffers &b, hipStream_t st);
+void cast_sorted_into(const hpt_scene *s, const LaunchForm &f, const float *org3,
+                      const float *dir3, int64_t n, int *out_id, float *out_t,
+                      const SortBuffers &b, hipStream_t st);
 }  // namespace hpt

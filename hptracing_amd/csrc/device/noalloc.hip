@@ -69,13 +69,14 @@ int hpt_cast_noalloc(hpt_scene *s, const float *org, const float *dir, int64_t n
         if (device != s->device)
             fail(HPT_ERR_INVALID, "hpt_cast_noalloc: the current device is not the scene's");
         unsigned char *base = scratch_on(device);
+        const LaunchForm f = s->form();
         hipStream_t st = (hipStream_t)stream;
         WorkCounter *wc = reinterpret_cast<WorkCounter *>(base);
         // The array keeps the previous call's contents: the counter starts
         // at zero for this launch.
         launch_zero_counter(wc, st);
         hip_check(hipGetLastError(), "k_zero_counter");
-        if (s->sort_rays && n >= kSortMinRays) {
+        if (f.sort_rays && n >= kSortMinRays) {
             unsigned char *q = base + kCounterBytes;
             SortBuffers b{};
             for (int k = 0; k < 2; ++k) {
@@ -100,11 +101,11 @@ int hpt_cast_noalloc(hpt_scene *s, const float *org, const float *dir, int64_t n
                                           " bytes of temporary storage, the scratch holds " +
                                           std::to_string(kSortTempBytes));
             b.wc = wc;
-            cast_sorted_into(s, org, dir, n, tri_id, t, b, st);
+            cast_sorted_into(s, f, org, dir, n, tri_id, t, b, st);
             return;
         }
-        launch_cast(s->dev, org, dir, nullptr, nullptr, CountRef{}, n, wc, tri_id, t, nullptr,
-                    nullptr, s->cast_grid, st);
+        launch_cast(f.dev, org, dir, nullptr, nullptr, CountRef{}, n, wc, tri_id, t, nullptr,
+                    nullptr, f.cast_grid, st);
         hip_check(hipGetLastError(), "k_cast launch");
     });
 }

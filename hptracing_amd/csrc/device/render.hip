@@ -309,6 +309,7 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
     }
     auto apply = [&](const hpt_tuning &t) {
         std::lock_guard<std::mutex> lock(s->mu);
+        std::lock_guard<std::mutex> tlock(s->tune_mu);
         s->dev.fast_div = t.fast_division;
         s->dev.traversal = t.traversal;
         s->dev.refill = t.refill_threshold;
@@ -449,11 +450,12 @@ int hpt_autotune_cast(hpt_scene *s, const float *org, const float *dir, int64_t 
                 KernelTimer timer(true, st);
                 timer.begin();
                 // as hpt_cast runs it, the sort of sort_rays included
-                if (s->sort_rays && m >= kSortMinRays)
-                    cast_sorted(s, org, dir, m, ids.as<int>(), ts.as<float>(), st);
+                const LaunchForm f = s->form();
+                if (f.sort_rays && m >= kSortMinRays)
+                    cast_sorted(s, f, org, dir, m, ids.as<int>(), ts.as<float>(), st);
                 else
-                    launch_cast(s->dev, org, dir, nullptr, nullptr, CountRef{}, m, wc.p,
-                                ids.as<int>(), ts.as<float>(), nullptr, nullptr, s->cast_grid,
+                    launch_cast(f.dev, org, dir, nullptr, nullptr, CountRef{}, m, wc.p,
+                                ids.as<int>(), ts.as<float>(), nullptr, nullptr, f.cast_grid,
                                 st);
                 timer.end();
                 hip_check(hipStreamSynchronize(st), "autotune cast");

@@ -138,8 +138,9 @@ struct hpt_server {
 
     void run();
     void admit_locked(std::vector<Request *> &admitted);
-    void start_requests(const std::vector<Request *> &admitted);
-    void one_pass(std::vector<Request *> &finished, std::vector<Request *> &progressed);
+    void start_requests(const LaunchForm &f, const std::vector<Request *> &admitted);
+    void one_pass(const LaunchForm &f, std::vector<Request *> &finished,
+                  std::vector<Request *> &progressed);
     void resolve(Request *r, int samples);
     void publish(const std::vector<Request *> &progressed);
     void finish(const std::vector<Request *> &finished);
@@ -161,7 +162,7 @@ void hpt_server::admit_locked(std::vector<Request *> &admitted) {
     }
 }
 
-void hpt_server::start_requests(const std::vector<Request *> &admitted) {
+void hpt_server::start_requests(const LaunchForm &f, const std::vector<Request *> &admitted) {
     hipStream_t st = stream;
     ViewParams *d_views = ws.views.as<ViewParams>();
     int *prim_count = ws.prim_count.as<int>();
@@ -193,9 +194,9 @@ void hpt_server::start_requests(const std::vector<Request *> &admitted) {
                   "upload view");
         hip_check(hipMemsetAsync(prim_count + r->slot, 0, sizeof(int), st), "memset");
         launch_camera(d_views, r->slot, r->R, prim, st);
-        launch_cast(scene->dev, nullptr, nullptr, prim.org + r->base, prim.dir + r->base,
-                    CountRef{}, r->R, ws.counter.as<WorkCounter>(), ws.prim_id.as<int>() + r->base,
-                    ws.prim_t.as<float>() + r->base, nullptr, nullptr, scene->cast_grid, st);
+        launch_cast(f.dev, nullptr, nullptr, prim.org + r->base, prim.dir + r->base, CountRef{},
+                    r->R, ws.counter.as<WorkCounter>(), ws.prim_id.as<int>() + r->base,
+                    ws.prim_t.as<float>() + r->base, nullptr, nullptr, f.cast_grid, st);
         launch_primary(d_views, r->slot, r->R, ws.prim_id.as<int>(), ws.prim_t.as<float>(),
                        ws.accum.as<float>(), ws.prim_hit.as<int2>(), ws.prim_list.as<int>(),
                        prim_count, st);
@@ -210,7 +211,8 @@ void hpt_server::start_requests(const std::vector<Request *> &admitted) {
     }
 }
 
-void hpt_server::one_pass(std::vector<Request *> &finished, std::vector<Request *> &progressed) {
+void hpt_server::one_pass(const LaunchForm &f, std::vector<Request *> &finished,
+                          std::vector<Request *> &progressed) {
     hipStream_t st = stream;
     std::vector<Segment> segs;
     PassDesc pd{0, 0, 0};
@@ -248,14 +250,14 @@ void hpt_server::one_pass(std::vector<Request *> &finished, std::vector<Request 
                       ws.hit_id.as<int>(), ws.hit_t.as<float>(), CountRef{live, pass, stride, 0}, st);
         for (int i = 0; i < depth; ++i) {
             if (i > 0)
-                launch_cast(scene->dev, nullptr, nullptr, cur.org, cur.dir,
+                launch_cast(f.dev, nullptr, nullptr, cur.org, cur.dir,
                             CountRef{live, pass, stride, i}, 0, ws.counter.as<WorkCounter>(),
                             ws.hit_id.as<int>(), ws.hit_t.as<float>(), nullptr, nullptr,
-                            scene->cast_grid, st);
-            launch_shade(scene->dev, d_views, i, CountRef{live, pass, stride, i}, cur,
+                            f.cast_grid, st);
+            launch_shade(f.dev, d_views, i, CountRef{live, pass, stride, i}, cur,
                          ws.hit_id.as<int>(), ws.hit_t.as<float>(), nxt,
                          CountRef{live, pass, stride, i + 1}, ws.accum.as<float>(),
-                         scene->shade_grid, st);
+                         f.shade_grid, st);
             std::swap(cur, nxt);
         }
         std::vector<int> lv(depth + 1);
@@ -338,8 +340,11 @@ void hpt_server::run() {
         }
         const double t0 = now_s();
         try {
-            if (!admitted.empty()) start_requests(admitted);
-            one_pass(finished, progressed);
+            // The scene's launch settings as they are now, for this macro
+            // pass (settings never change a result).
+            const LaunchForm f = scene->form();
+            if (!admitted.empty()) start_requests(f, admitted);
+            one_pass(f, finished, progressed);
             publish(progressed);
             finish(finished);
         } catch (const Fail &e) {

@@ -105,8 +105,9 @@ size_t sort_temp_bytes(int64_t n, hipStream_t st) {
     return bytes;
 }
 
-void cast_sorted_into(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
-                      float *out_t, const SortBuffers &b, hipStream_t st) {
+void cast_sorted_into(const hpt_scene *s, const LaunchForm &f, const float *org3,
+                      const float *dir3, int64_t n, int *out_id, float *out_t,
+                      const SortBuffers &b, hipStream_t st) {
     float3 lo, scale;
     float *lo_p = &lo.x, *scale_p = &scale.x;
     for (int k = 0; k < 3; ++k) {
@@ -128,15 +129,15 @@ void cast_sorted_into(hpt_scene *s, const float *org3, const float *dir3, int64_
     const int32_t *order = idx.Current();
     k_gather_rays<<<grid, kBlock, 0, st>>>(org3, dir3, order, n, b.org4, b.dir4);
     hip_check(hipGetLastError(), "k_gather_rays");
-    launch_cast(s->dev, nullptr, nullptr, b.org4, b.dir4, CountRef{}, n, b.wc, b.ids, b.t, nullptr,
-                nullptr, s->cast_grid, st);
+    launch_cast(f.dev, nullptr, nullptr, b.org4, b.dir4, CountRef{}, n, b.wc, b.ids, b.t, nullptr,
+                nullptr, f.cast_grid, st);
     hip_check(hipGetLastError(), "k_cast launch");
     k_scatter_hits<<<grid, kBlock, 0, st>>>(order, b.ids, b.t, n, out_id, out_t);
     hip_check(hipGetLastError(), "k_scatter_hits");
 }
 
-void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, int *out_id,
-                 float *out_t, hipStream_t st) {
+void cast_sorted(const hpt_scene *s, const LaunchForm &f, const float *org3, const float *dir3,
+                 int64_t n, int *out_id, float *out_t, hipStream_t st) {
     AsyncBuffers buf(st);
     SortBuffers b{};
     for (int k = 0; k < 2; ++k) {
@@ -151,7 +152,7 @@ void cast_sorted(hpt_scene *s, const float *org3, const float *dir3, int64_t n, 
     b.temp = buf.get<uint8_t>(b.temp_bytes);
     StreamCounter wc(st);
     b.wc = wc.p;
-    cast_sorted_into(s, org3, dir3, n, out_id, out_t, b, st);
+    cast_sorted_into(s, f, org3, dir3, n, out_id, out_t, b, st);
 }
 
 }  // namespace hpt
