@@ -220,6 +220,9 @@ typedef struct {
                                     sort on the caller's stream
                                     (hpt_cast_noalloc: a static array)    */
 } hpt_tuning;
+/* Setting the tuning waits for a render running on the scene; casts and
+ * servers on the scene launch with a consistent copy of the settings taken
+ * when each launch (or server macro pass) starts. */
 int hpt_scene_get_tuning(const hpt_scene *scene, hpt_tuning *out);
 int hpt_scene_set_tuning(hpt_scene *scene, const hpt_tuning *tuning);
 
