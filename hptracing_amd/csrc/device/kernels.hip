@@ -550,7 +550,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             for (int k = 0; k < node_steps; ++k) {
                 const bool walk = ray >= 0 && !done && pending < QN;
                 const uint64_t wm = __ballot(walk);
-                if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
+                if (wm == 0 || (k > 0 && (int)__popcll(wm) < walk_min)) break;
                 if (walk) {
                     float4 lo, hi;
                     fetch(idx, lo, hi);
@@ -561,7 +561,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
             for (int k = 0; k < node_steps; ++k) {
                 const bool walk = ray >= 0 && !done && pending < QN;
                 const uint64_t wm = __ballot(walk);
-                if (wm == 0 || (k > 0 && __popcll(wm) < walk_min)) break;
+                if (wm == 0 || (k > 0 && (int)__popcll(wm) < walk_min)) break;
                 if (walk) {
                     if (WIN > 1) {
                         // The walk only moves forward, so every node of the
