@@ -124,14 +124,26 @@ def full_scene():
     return KDTree(path, os.path.dirname(path) + "/")
 
 
+@pytest.fixture(scope="module")
+def serving_scene():
+    """The serving workload's scene: the 1080p scene with the water grid
+    raised to 2 M triangles (2 221 316 triangles)."""
+    import tempfile
+    from hptracing_amd.scene import KDTree
+    path = scenes.cached(os.path.join(tempfile.gettempdir(), "hpt_scenes"), scenes.write_full, 0,
+                         water_side=1000)
+    return KDTree(path, os.path.dirname(path) + "/")
+
+
 @needs_ref
-@pytest.mark.parametrize("which", ["soup200k", "full1080p"])
+@pytest.mark.parametrize("which", ["soup200k", "full1080p", "serving2m"])
 def test_full_size_batch_against_reference_kernel(which, request):
     """2^22 rays through the ray-cast workload's 200 000-triangle soup
-    (origins uniform in its box) and through the 1080p render's scene
-    (origins uniform in the room, as bounce rays start), directions uniform
-    on the sphere: every hit triangle and every distance equals the
-    reference kernel's (the default arithmetic).  Both kernels' device times
+    (origins uniform in its box), through the 1080p render's scene and
+    through the serving workload's 2.2 M-triangle scene (origins uniform in
+    the room, as bounce rays start), directions uniform on the sphere: every
+    hit triangle and every distance equals the reference kernel's (the
+    default arithmetic).  Both kernels' device times
     on the same batch are printed (the reference kernel as compiled for the
     parity comparison: -O3, no contraction, correctly rounded division)."""
     import torch
@@ -141,7 +153,7 @@ def test_full_size_batch_against_reference_kernel(which, request):
         pts = kd.points[:, :3]
         lo, hi = pts.min(0), pts.max(0)
     else:
-        kd = request.getfixturevalue("full_scene")
+        kd = request.getfixturevalue("full_scene" if which == "full1080p" else "serving_scene")
         lo, hi = np.full(3, 0.05), np.full(3, scenes.ROOM - 0.05)
     org, d = scenes.random_rays(1 << 22, lo, hi, seed=41)
     r = TraceRunner(kd)  # default arithmetic
