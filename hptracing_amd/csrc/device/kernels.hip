@@ -403,9 +403,10 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
 //         walks until it has queued Q non-empty leaves (in LDS) or its walk
 //         ends; then the (ray, triangle) pairs of every lane's queue are
 //         numbered by a wave prefix sum and tested 64 at a time, one pair
-//         per lane, each tester reading its ray from the owning lane; the
-//         owner folds its distances from LDS in queue order, which is the
-//         reference's order, so best and ties come out the same.  A leaf
+//         per lane, each tester reading its ray from the owning lane; a
+//         segmented wave scan folds each owner's distances in position
+//         order, which is its queue order and the reference's order, so
+//         best and ties come out the same.  A leaf
 //         round costs ceil(sum of counts / 64) tests instead of the
 //         largest count, and lanes keep walking while their queue fills.
 //  KEYED  (FLAT only) the leaf round folds through 64-bit keys in LDS

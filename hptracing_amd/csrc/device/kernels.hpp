@@ -58,7 +58,8 @@ struct DevScene {
                         // bit 2 leaf tests spread over the wave (not with bit 1),
                         // bits 3-4 (with bit 2) leaf queue depth 1 << value,
                         // bit 5 six-wave register budget, bit 6 keyed leaf fold,
-                        // bit 7 per-triangle records through leaf ids
+                        // bit 7 per-triangle records through leaf ids,
+                        // bits 8-9 (with bit 0) node window of 2 + value
 };
 
 struct Paths {
