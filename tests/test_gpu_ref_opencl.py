@@ -179,3 +179,45 @@ def test_full_size_batch_against_reference_kernel(which, request):
           f"hits {(ids >= 0).sum()}")
     assert (ours_geo == ref_geo).all()
     assert np.array_equal(t.view(np.int32), ref_t.view(np.int32))
+
+
+def _degenerate_rays(seed):
+    """Axis-parallel rays, rays with one or two zero direction components,
+    origins on integer planes (the room's walls and boxes), a zero
+    direction, subnormal and huge components (the division form's cases)."""
+    rng = np.random.default_rng(seed)
+    n = 20000
+    org = rng.uniform(0, 10, (n, 3)).astype(np.float32)
+    org[: n // 8, 1] = 0.0
+    org[n // 8: n // 4] = np.round(org[n // 8: n // 4])
+    d = np.zeros((n, 3), np.float32)
+    axis = rng.integers(0, 3, n)
+    d[np.arange(n), axis] = rng.choice([-1.0, 1.0], n)
+    d[n // 2:] = rng.normal(size=(n - n // 2, 3)).astype(np.float32)
+    d[n // 2: n // 2 + 2000, 0] = 0.0
+    d[n // 2 + 2000: n // 2 + 2100] = 0.0
+    d[n // 2 + 2100: n // 2 + 2200, 0] = 1e-40
+    d[n // 2 + 2200: n // 2 + 2300] *= 1e30
+    return org, d
+
+
+@needs_ref
+@pytest.mark.parametrize("which", ["room", "soup"])
+def test_near_ties_against_reference_kernel(which, request):
+    """The slab test's near-ties and special cases against the reference's
+    own kernel: rays aimed at corners, edges and faces of the scene's KD
+    boxes, rays starting on box faces (some with a zero direction
+    component), and axis-parallel, zero, subnormal and huge directions.
+    These are where the certified reciprocal form falls back to the
+    division form, and where the start-inside rule and IEEE minNum/maxNum on
+    NaN quotients decide; every hit triangle and distance equals the
+    reference kernel's."""
+    from test_gpu_cast import _grazing_rays
+    kd = request.getfixturevalue("room" if which == "room" else "soup_small")
+    org, d = _grazing_rays(kd, 20000, seed=51)
+    if which == "room":
+        o2, d2 = _degenerate_rays(52)
+        org, d = np.concatenate([org, o2]), np.concatenate([d, d2])
+    same, rel, exact_t = compare(kd, org, d)
+    assert same.all()
+    assert exact_t == 1.0
