@@ -165,6 +165,10 @@ typedef struct hpt_scene hpt_scene;
  * (trace_runner.cpp:41-108).  Copies and repacks every array into HBM; the
  * descriptor may be freed afterwards. */
 int hpt_scene_create(const hpt_scene_desc *desc, hpt_scene **out);
+/* Frees the scene's device memory.  Every server created on the scene must
+ * be destroyed first, no other call on the scene may still be running, and
+ * casts enqueued on the caller's streams must have completed: destroy waits
+ * only for the scene's own stream (renders). */
 int hpt_scene_destroy(hpt_scene *scene);
 
 typedef struct {
