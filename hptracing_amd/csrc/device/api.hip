@@ -250,9 +250,9 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         // which fuses them (HPT_ARITH_ROCM_OPENCL); its hit distances are
         // then reproduced bit for bit.
         ds.arith_ocl = 1;
-        ds.refill = 48;
+        ds.refill = 40;
         ds.node_steps = 24;
-        ds.fetch = 445;
+        ds.fetch = 1469;
         ds.walk_min = 0;
 
         int cus = 0;

@@ -413,7 +413,10 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
 //         (keyed_leaf_round) and testers read rays from LDS.
 //  IND    (FLAT only) leaf rounds read the records once per triangle
 //         through the leaf entries' triangle ids (tri_record).
-template <bool PATHS, bool FAST, int WIN, bool PREF, int Q, bool KEYED, bool IND>
+//  LINE   the window ends at the end of the 128-byte line (four nodes) that
+//         holds idx: a trip reads only nodes of one line, so it touches one
+//         line and one or two 64-byte sectors instead of up to two lines.
+template <bool PATHS, bool FAST, int WIN, bool PREF, int Q, bool KEYED, bool IND, bool LINE>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
                                             const float *dir3, const float4 *org4,
                                             const float4 *dir4, int64_t n, WorkCounter *wc,
@@ -569,14 +572,19 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                         // window that it reaches is visited in order; the
                         // loads past the last node repeat that node.
                         float4 lo[WIN], hi[WIN];
+                        // nodes of the window: all WIN, or (LINE) those up
+                        // to the end of idx's line
+                        const int len = LINE ? min(WIN, 4 - (idx & 3)) : WIN;
 #pragma unroll
                         for (int j = 0; j < WIN; ++j)
-                            fetch(idx + j < nn ? idx + j : nn - 1, lo[j], hi[j]);
+                            if (!LINE || j < len)
+                                fetch(idx + j < nn ? idx + j : nn - 1, lo[j], hi[j]);
                         const int first = idx;
                         visit(lo[0], hi[0]);
 #pragma unroll
                         for (int j = 1; j < WIN; ++j)
-                            if (!done && pending < QN && idx == first + j) visit(lo[j], hi[j]);
+                            if (!done && pending < QN && idx == first + j && (!LINE || j < len))
+                                visit(lo[j], hi[j]);
                     } else {
                         float4 lo, hi;
                         fetch(idx, lo, hi);
@@ -633,7 +641,8 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
 // queue depth 1 << ((FETCH >> 3) & 3); bit 32 compiles the form for six
 // waves per SIMD (80 VGPRs) instead of eight (64); bit 64 selects the keyed
 // leaf-round fold, bit 128 the indirect (per-triangle) records; with bit 1,
-// bits 8-9 widen the node window from 2 to 2 + ((FETCH >> 8) & 3).
+// bits 8-9 widen the node window from 2 to 2 + ((FETCH >> 8) & 3); bit 10
+// ends the window at the end of idx's 128-byte line.
 // The keyed forms hold 21 KB (queue of 4) or 29 KB (queue of 8) of LDS per
 // block, so at most 7 or 5 blocks fit in a CU's 160 KB.
 constexpr int cast_waves_per_eu(int fetch) {
@@ -649,8 +658,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     const int64_t n = n_dev.base ? (int64_t)*n_dev.ptr() : n_host;
     constexpr int Q = (FETCH & 4) ? (1 << ((FETCH >> 3) & 3)) : 0;
     constexpr int WIN = (FETCH & 1) ? 2 + ((FETCH >> 8) & 3) : 1;
-    cast_refill<PATHS, FAST, WIN, (FETCH & 6) == 2, Q, (FETCH & 64) != 0,
-                (FETCH & 128) != 0>(sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
+    cast_refill<PATHS, FAST, WIN, (FETCH & 6) == 2, Q, (FETCH & 64) != 0, (FETCH & 128) != 0,
+                (FETCH & 1024) != 0>(sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
     wave_done(wc);
 }
 
@@ -1183,6 +1192,8 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
     case 445: HPT_REFILL(P, F, 445); break;                                                      \
     case 669: HPT_REFILL(P, F, 669); break;                                                      \
     case 701: HPT_REFILL(P, F, 701); break;                                                      \
+    case 1469: HPT_REFILL(P, F, 1469); break;                                                    \
+    case 1725: HPT_REFILL(P, F, 1725); break;                                                    \
     default: HPT_REFILL(P, F, 0); break;                                                         \
     }
         if (paths) { if (fast) { HPT_REFILL_M(true, true) } else { HPT_REFILL_M(true, false) } }

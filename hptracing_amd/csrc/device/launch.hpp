@@ -41,14 +41,15 @@ int occupancy_blocks_shade();
 // and (without node pairs) bit 6 (64) folds the leaf rounds through keys in
 // LDS; bit 7 (128)
 // reads the triangle records once per triangle through the leaf entries'
-// ids; bits 8-9 (with 1) widen the node window.  These are the compiled
-// forms.
+// ids; bits 8-9 (with 1) widen the node window, and bit 10 (1024) ends it
+// at the end of the 128-byte line that holds the first node.  These are
+// the compiled forms.
 inline bool valid_fetch(int f) {
     switch (f) {
     case 0: case 1: case 2: case 3: case 4: case 5: case 12: case 13: case 20: case 21:
     case 28: case 29: case 52: case 60: case 84: case 92: case 116: case 124: case 148:
     case 149: case 156: case 157: case 189: case 220: case 252: case 413: case 445:
-    case 669: case 701:
+    case 669: case 701: case 1469: case 1725:
         return true;
     default:
         return false;

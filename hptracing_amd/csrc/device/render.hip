@@ -347,7 +347,8 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
         if (best.traversal == 2) {
             try_knob([](hpt_tuning &t, int v) { t.fetch = v; },
                      {0, 1, 4, 5, 12, 13, 20, 21, 28, 29, 52, 60, 84, 92, 116, 124, 148,
-                      149, 156, 157, 189, 220, 252, 413, 445, 669, 701});
+                      149, 156, 157, 189, 220, 252, 413, 445, 669, 701, 1469,
+                      1725});
             try_knob([](hpt_tuning &t, int v) { t.refill_threshold = v; },
                      {8, 16, 24, 32, 48, 64});
             try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {8, 16, 24, 32, 48, 64});

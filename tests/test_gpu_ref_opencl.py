@@ -61,7 +61,8 @@ def compare(kd, org, d, rocm_opencl=None):
     ours_geo = np.where(ids[:, None] >= 0, kd.geometries[np.maximum(ids, 0)], -1)
     same = (ours_geo == ref_geo).all(axis=1)
     hit = ids >= 0
-    rel = np.abs(t[hit & same] - ref_t[hit & same]) / np.maximum(np.abs(ref_t[hit & same]), 1e-30)
+    with np.errstate(invalid="ignore"):  # inf - inf where both distances are infinite
+        rel = np.abs(t[hit & same] - ref_t[hit & same]) / np.maximum(np.abs(ref_t[hit & same]), 1e-30)
     exact_t = np.mean(t[hit & same].view(np.int32) == ref_t[hit & same].view(np.int32))
     print(f"\n[ref-opencl] arith={'literal' if rocm_opencl is False else 'rocm-opencl'} rays {len(ids)} "
           f"hits {hit.sum()} id-mismatch {(~same).sum()} "
