@@ -252,7 +252,9 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.arith_ocl = 1;
         ds.refill = 40;
         ds.node_steps = 24;
-        ds.fetch = 1469;
+        // Buffer loads (fetch bit 2048) address nodes by 32-bit offsets:
+        // node arrays under 2 GiB.
+        ds.fetch = hpt::fetch_fits(3517, nn) ? 3517 : 1469;
         ds.walk_min = 0;
 
         int cus = 0;
@@ -315,7 +317,8 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
             t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64 ||
             t->traversal < 0 || t->traversal > 2 || t->refill_threshold < 1 ||
             t->refill_threshold > 64 || t->node_steps < 1 || t->node_steps > 4096 ||
-            !hpt::valid_fetch(t->fetch) || t->walk_min < 0 || t->walk_min > 64 ||
+            !hpt::valid_fetch(t->fetch) || !hpt::fetch_fits(t->fetch, s->n_nodes) ||
+            t->walk_min < 0 || t->walk_min > 64 ||
             t->sort_rays < 0 || t->sort_rays > 1)
             fail(HPT_ERR_INVALID, "tuning value out of range");
         std::lock_guard<std::mutex> lock(s->mu);

@@ -416,7 +416,12 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
 //  LINE   the window ends at the end of the 128-byte line (four nodes) that
 //         holds idx: a trip reads only nodes of one line, so it touches one
 //         line and one or two 64-byte sectors instead of up to two lines.
-template <bool PATHS, bool FAST, int WIN, bool PREF, int Q, bool KEYED, bool IND, bool LINE>
+//  BUF    nodes are read with buffer loads: 32-bit byte offsets (no 64-bit
+//         address arithmetic) and a range check that returns zeros past the
+//         last node (such window slots are never visited), so the window
+//         needs no clamp.  The node array must be under 2 GiB (fetch_fits).
+template <bool PATHS, bool FAST, int WIN, bool PREF, int Q, bool KEYED, bool IND, bool LINE,
+          bool BUF>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
                                             const float *dir3, const float4 *org4,
                                             const float4 *dir4, int64_t n, WorkCounter *wc,
@@ -437,8 +442,20 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
     bool rcp = false, done = true;
     // Node q's two halves (box_start + a, box_end + b).
     auto fetch = [&](int q, float4 &lo, float4 &hi) {
-        lo = sc.nodes[2 * q];
-        hi = sc.nodes[2 * q + 1];
+        if constexpr (BUF) {
+            const __amdgpu_buffer_rsrc_t nodes = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)sc.nodes, (short)0, (int)((uint32_t)nn * 32u), 0x00020000);
+            const uint32_t off = (uint32_t)q << 5;
+            const auto a = __builtin_amdgcn_raw_buffer_load_b128(nodes, (int)off, 0, 0);
+            const auto b = __builtin_amdgcn_raw_buffer_load_b128(nodes, (int)(off + 16u), 0, 0);
+            lo = make_float4(__int_as_float(a[0]), __int_as_float(a[1]), __int_as_float(a[2]),
+                             __int_as_float(a[3]));
+            hi = make_float4(__int_as_float(b[0]), __int_as_float(b[1]), __int_as_float(b[2]),
+                             __int_as_float(b[3]));
+        } else {
+            lo = sc.nodes[2 * q];
+            hi = sc.nodes[2 * q + 1];
+        }
     };
     // pending: leaves held for the next leaf round (FLAT: queued in LDS,
     // pcnt = their triangle total; otherwise pcnt / poff of the one leaf).
@@ -578,7 +595,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
 #pragma unroll
                         for (int j = 0; j < WIN; ++j)
                             if (!LINE || j < len)
-                                fetch(idx + j < nn ? idx + j : nn - 1, lo[j], hi[j]);
+                                fetch(BUF || idx + j < nn ? idx + j : nn - 1, lo[j], hi[j]);
                         const int first = idx;
                         visit(lo[0], hi[0]);
 #pragma unroll
@@ -642,7 +659,8 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
 // waves per SIMD (80 VGPRs) instead of eight (64); bit 64 selects the keyed
 // leaf-round fold, bit 128 the indirect (per-triangle) records; with bit 1,
 // bits 8-9 widen the node window from 2 to 2 + ((FETCH >> 8) & 3); bit 10
-// ends the window at the end of idx's 128-byte line.
+// ends the window at the end of idx's 128-byte line; bit 11 reads the nodes
+// with buffer loads.
 // The keyed forms hold 21 KB (queue of 4) or 29 KB (queue of 8) of LDS per
 // block, so at most 7 or 5 blocks fit in a CU's 160 KB.
 constexpr int cast_waves_per_eu(int fetch) {
@@ -659,7 +677,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     constexpr int Q = (FETCH & 4) ? (1 << ((FETCH >> 3) & 3)) : 0;
     constexpr int WIN = (FETCH & 1) ? 2 + ((FETCH >> 8) & 3) : 1;
     cast_refill<PATHS, FAST, WIN, (FETCH & 6) == 2, Q, (FETCH & 64) != 0, (FETCH & 128) != 0,
-                (FETCH & 1024) != 0>(sc, org3, dir3, org4, dir4, n, wc, out_id, out_t);
+                (FETCH & 1024) != 0, (FETCH & 2048) != 0>(sc, org3, dir3, org4, dir4, n, wc,
+                                                          out_id, out_t);
     wave_done(wc);
 }
 
@@ -1194,6 +1213,8 @@ void launch_cast(const DevScene &sc, const float *org3, const float *dir3, const
     case 701: HPT_REFILL(P, F, 701); break;                                                      \
     case 1469: HPT_REFILL(P, F, 1469); break;                                                    \
     case 1725: HPT_REFILL(P, F, 1725); break;                                                    \
+    case 3517: HPT_REFILL(P, F, 3517); break;                                                    \
+    case 3773: HPT_REFILL(P, F, 3773); break;                                                    \
     default: HPT_REFILL(P, F, 0); break;                                                         \
     }
         if (paths) { if (fast) { HPT_REFILL_M(true, true) } else { HPT_REFILL_M(true, false) } }

@@ -41,19 +41,26 @@ int occupancy_blocks_shade();
 // and (without node pairs) bit 6 (64) folds the leaf rounds through keys in
 // LDS; bit 7 (128)
 // reads the triangle records once per triangle through the leaf entries'
-// ids; bits 8-9 (with 1) widen the node window, and bit 10 (1024) ends it
-// at the end of the 128-byte line that holds the first node.  These are
-// the compiled forms.
+// ids; bits 8-9 (with 1) widen the node window, bit 10 (1024) ends it at
+// the end of the 128-byte line that holds the first node, and bit 11 (2048)
+// reads the nodes with buffer loads (fetch_fits).  These are the compiled
+// forms.
 inline bool valid_fetch(int f) {
     switch (f) {
     case 0: case 1: case 2: case 3: case 4: case 5: case 12: case 13: case 20: case 21:
     case 28: case 29: case 52: case 60: case 84: case 92: case 116: case 124: case 148:
     case 149: case 156: case 157: case 189: case 220: case 252: case 413: case 445:
-    case 669: case 701: case 1469: case 1725:
+    case 669: case 701: case 1469: case 1725: case 3517: case 3773:
         return true;
     default:
         return false;
     }
+}
+
+// The buffer-load forms address nodes by 32-bit byte offsets, the window's
+// slots past the last node included: node arrays under 2 GiB.
+inline bool fetch_fits(int f, int64_t n_nodes) {
+    return !(f & 2048) || (n_nodes + 4) * 32 <= (int64_t)INT32_MAX;
 }
 
 }  // namespace hpt

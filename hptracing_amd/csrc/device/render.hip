@@ -332,7 +332,7 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
         for (int v : values) {
             hpt_tuning t = start;
             set(t, v);
-            if (!valid_fetch(t.fetch)) continue;
+            if (!valid_fetch(t.fetch) || !fetch_fits(t.fetch, s->n_nodes)) continue;
             // A knob moves only for a gain above the run-to-run noise.
             const double ms = timed(t, true);
             if (ms < best_ms * 0.99) {
@@ -348,7 +348,7 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
             try_knob([](hpt_tuning &t, int v) { t.fetch = v; },
                      {0, 1, 4, 5, 12, 13, 20, 21, 28, 29, 52, 60, 84, 92, 116, 124, 148,
                       149, 156, 157, 189, 220, 252, 413, 445, 669, 701, 1469,
-                      1725});
+                      1725, 3517, 3773});
             try_knob([](hpt_tuning &t, int v) { t.refill_threshold = v; },
                      {8, 16, 24, 32, 48, 64});
             try_knob([](hpt_tuning &t, int v) { t.node_steps = v; }, {8, 16, 24, 32, 48, 64});

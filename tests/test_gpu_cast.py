@@ -32,7 +32,8 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                         (1, 2, 48, 32, 149), (1, 2, 48, 32, 252), (0, 2, 8, 64, 220, 16),
                         (1, 2, 48, 24, 189), (0, 2, 1, 1, 189), (1, 2, 48, 24, 413),
                         (0, 2, 1, 1, 445), (1, 2, 48, 24, 669), (1, 2, 16, 64, 701, 24),
-                        (1, 2, 48, 24, 1469), (0, 2, 1, 1, 1469), (1, 2, 16, 64, 1725, 24)],
+                        (1, 2, 48, 24, 1469), (0, 2, 1, 1, 1469), (1, 2, 16, 64, 1725, 24),
+                        (1, 2, 40, 24, 3517), (0, 2, 1, 1, 3517), (1, 2, 16, 64, 3773, 24)],
                 ids=["ieee-refill", "rcp-refill-every-step", "ieee-refill-64", "rcp-phases",
                      "ieee-phases", "rcp-interleaved", "ieee-interleaved", "rcp-refill-pairs",
                      "rcp-refill-prefetch", "ieee-refill-pairs-prefetch",
@@ -51,7 +52,9 @@ def _runner(kd, fast=(0, 2, 16, 16, 0)):
                      "rcp-flat-q8-window3-indirect", "ieee-flat-q8-window3-wide-every-step",
                      "rcp-flat-q8-window4-indirect", "rcp-flat-q8-window4-wide-walk24",
                      "rcp-flat-q8-window3-line-indirect", "ieee-flat-q8-window3-line-every-step",
-                     "rcp-flat-q8-window4-line-wide-walk24"])
+                     "rcp-flat-q8-window4-line-wide-walk24",
+                     "rcp-flat-q8-window3-line-buffer", "ieee-flat-q8-window3-line-buffer-every-step",
+                     "rcp-flat-q8-window4-line-buffer-wide-walk24"])
 def fast(request):
     """(fast_division, traversal, refill_threshold, node_steps, fetch[,
     walk_min]) tunings: every one must give the same bits."""
@@ -183,7 +186,7 @@ def test_autotune_cast_keeps_results(soup_small):
     assert torch.equal(t0.view(torch.int32), t1.view(torch.int32))
 
 
-@pytest.mark.parametrize("fetch", [1469, 445, 157, 220, 0])
+@pytest.mark.parametrize("fetch", [3517, 1469, 445, 157, 220, 0])
 def test_cast_sorted_batches(room, soup_small, oracle_mod, fetch):
     """sort_rays casts a batch of at least 65536 rays in the Morton order of
     its origins and writes the results back in the caller's order: bit for

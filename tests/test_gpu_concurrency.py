@@ -126,7 +126,7 @@ def test_tuning_changes_while_casts_and_a_server_run(room):
     torch.cuda.synchronize()
     ids0, t0 = ids0.cpu().numpy(), t0.cpu().numpy()
     forms = [dict(fetch=f, cast_blocks_per_cu=b, node_steps=ns, refill_threshold=rf, sort_rays=sr)
-             for f, b, ns, rf, sr in [(1469, 6, 24, 48, 0), (157, 5, 16, 32, 1), (220, 4, 64, 64, 1),
+             for f, b, ns, rf, sr in [(3517, 6, 24, 40, 0), (1469, 6, 24, 48, 1), (157, 5, 16, 32, 1), (220, 4, 64, 64, 1),
                                       (5, 8, 8, 16, 0), (92, 5, 32, 48, 1)]]
     stop = threading.Event()
     errors = []
