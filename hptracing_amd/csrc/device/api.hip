@@ -231,6 +231,7 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         ds.tri_recs = s->tri_recs.as<float4>();
         ds.leaf_ids = s->leaf_ids.as<int32_t>();
         ds.n_entries = (int32_t)std::max<int64_t>(entries, 1);
+        ds.n_recs = (int32_t)std::max<int64_t>(nt, 1);
         ds.points = s->points.as<float4>();
         ds.normals = s->normals.as<float4>();
         ds.n_normals = (int32_t)d->n_normals;
@@ -250,11 +251,11 @@ int hpt_scene_create(const hpt_scene_desc *d, hpt_scene **out) {
         // which fuses them (HPT_ARITH_ROCM_OPENCL); its hit distances are
         // then reproduced bit for bit.
         ds.arith_ocl = 1;
-        ds.refill = 40;
+        ds.refill = 32;
         ds.node_steps = 24;
-        // Buffer loads (fetch bit 2048) address nodes by 32-bit offsets:
-        // node arrays under 2 GiB.
-        ds.fetch = hpt::fetch_fits(3517, nn) ? 3517 : 1469;
+        // Buffer loads (fetch bit 2048) address by 32-bit offsets: node,
+        // leaf-entry and record arrays under 2 GiB.
+        ds.fetch = hpt::fetch_fits(3517, ds) ? 3517 : 1469;
         ds.walk_min = 0;
 
         int cus = 0;
@@ -317,7 +318,7 @@ int hpt_scene_set_tuning(hpt_scene *s, const hpt_tuning *t) {
             t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 64 ||
             t->traversal < 0 || t->traversal > 2 || t->refill_threshold < 1 ||
             t->refill_threshold > 64 || t->node_steps < 1 || t->node_steps > 4096 ||
-            !hpt::valid_fetch(t->fetch) || !hpt::fetch_fits(t->fetch, s->n_nodes) ||
+            !hpt::valid_fetch(t->fetch) || !hpt::fetch_fits(t->fetch, s->dev) ||
             t->walk_min < 0 || t->walk_min > 64 ||
             t->sort_rays < 0 || t->sort_rays > 1)
             fail(HPT_ERR_INVALID, "tuning value out of range");

@@ -230,6 +230,40 @@ __device__ __forceinline__ const float4 *tri_record(const DevScene &sc, int at) 
     return IND ? sc.tri_recs + 3 * (int64_t)sc.leaf_ids[at] : sc.tris + 3 * (int64_t)at;
 }
 
+// Buffer loads: a resource over `bytes` bytes from p (under 2 GiB) takes
+// 32-bit byte offsets, with no 64-bit address arithmetic per load, and an
+// offset at or past the end reads zeros.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float4 buffer_float4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return make_float4(__int_as_float(v[0]), __int_as_float(v[1]), __int_as_float(v[2]),
+                       __int_as_float(v[3]));
+}
+
+// The three float4 of leaf entry `at`'s record (tri_record); BUF reads the
+// indirect form's record number and record with buffer loads.
+template <bool IND, bool BUF>
+__device__ __forceinline__ void load_record(const DevScene &sc, int at, float4 &r0, float4 &r1,
+                                            float4 &r2) {
+    if constexpr (IND && BUF) {
+        const int k = __builtin_amdgcn_raw_buffer_load_b32(
+            buffer_over(sc.leaf_ids, (uint32_t)sc.n_entries * 4u), (int)((uint32_t)at << 2), 0, 0);
+        const __amdgpu_buffer_rsrc_t recs = buffer_over(sc.tri_recs, (uint32_t)sc.n_recs * 48u);
+        const uint32_t off = (uint32_t)k * 48u;
+        r0 = buffer_float4(recs, off);
+        r1 = buffer_float4(recs, off + 16u);
+        r2 = buffer_float4(recs, off + 32u);
+    } else {
+        const float4 *rec = tri_record<IND>(sc, at);
+        r0 = rec[0];
+        r1 = rec[1];
+        r2 = rec[2];
+    }
+}
+
 // Leaf entry of position k (0 <= k < the lane's count) of the work of the
 // lane whose queue is in column `col`, with `nq` leaves queued: the slots
 // from nq on hold stale or never-written values and do not take part.  The
@@ -251,7 +285,7 @@ __device__ __forceinline__ int queue_record(const LeafQueue<Q> &lq, int col, int
 // is the triangles of its `nq` queued leaves in queue order, `cnt` in all
 // (0 and 0 if none); best / best_id are folded in place.  All 64 lanes must
 // call it.
-template <int Q, bool IND>
+template <int Q, bool IND, bool BUF>
 __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
                                                 int lane, int nq, int cnt, V3 sp, V3 d,
                                                 float &best, int &best_id) {
@@ -283,8 +317,8 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
             // Position j - o_excl of the owner's work (0 <= it < the owner's
             // count, the sum of the counts of its o_nq queued leaves).
             const int at = queue_record<Q>(lq, w + own, o_nq, j - o_excl, sc.n_entries);
-            const float4 *rec = tri_record<IND>(sc, at);
-            const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+            float4 r0, r1, r2;
+            load_record<IND, BUF>(sc, at, r0, r1, r2);
             r = sc.arith_ocl ? tri_hit_ocl(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2))
                              : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
             id = __float_as_int(r0.w);
@@ -416,10 +450,11 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
 //  LINE   the window ends at the end of the 128-byte line (four nodes) that
 //         holds idx: a trip reads only nodes of one line, so it touches one
 //         line and one or two 64-byte sectors instead of up to two lines.
-//  BUF    nodes are read with buffer loads: 32-bit byte offsets (no 64-bit
-//         address arithmetic) and a range check that returns zeros past the
-//         last node (such window slots are never visited), so the window
-//         needs no clamp.  The node array must be under 2 GiB (fetch_fits).
+//  BUF    nodes, and (IND) the leaf entries' record numbers and the
+//         records, are read with buffer loads: 32-bit byte offsets (no
+//         64-bit address arithmetic) and a range check that returns zeros
+//         past the last node (such window slots are never visited), so the
+//         window needs no clamp.  Each array must be under 2 GiB (fetch_fits).
 template <bool PATHS, bool FAST, int WIN, bool PREF, int Q, bool KEYED, bool IND, bool LINE,
           bool BUF>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
@@ -443,15 +478,10 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
     // Node q's two halves (box_start + a, box_end + b).
     auto fetch = [&](int q, float4 &lo, float4 &hi) {
         if constexpr (BUF) {
-            const __amdgpu_buffer_rsrc_t nodes = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)sc.nodes, (short)0, (int)((uint32_t)nn * 32u), 0x00020000);
+            const __amdgpu_buffer_rsrc_t nodes = buffer_over(sc.nodes, (uint32_t)nn * 32u);
             const uint32_t off = (uint32_t)q << 5;
-            const auto a = __builtin_amdgcn_raw_buffer_load_b128(nodes, (int)off, 0, 0);
-            const auto b = __builtin_amdgcn_raw_buffer_load_b128(nodes, (int)(off + 16u), 0, 0);
-            lo = make_float4(__int_as_float(a[0]), __int_as_float(a[1]), __int_as_float(a[2]),
-                             __int_as_float(a[3]));
-            hi = make_float4(__int_as_float(b[0]), __int_as_float(b[1]), __int_as_float(b[2]),
-                             __int_as_float(b[3]));
+            lo = buffer_float4(nodes, off);
+            hi = buffer_float4(nodes, off + 16u);
         } else {
             lo = sc.nodes[2 * q];
             hi = sc.nodes[2 * q + 1];
@@ -617,7 +647,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                 keyed_leaf_round<QN, IND>(sc, lq, rs, s_key, lane, own ? pending : 0,
                                      own ? pcnt : 0, best, best_id);
             else
-                flat_leaf_round<QN, IND>(sc, lq, lane, own ? pending : 0, own ? pcnt : 0, sp, d,
+                flat_leaf_round<QN, IND, BUF>(sc, lq, lane, own ? pending : 0, own ? pcnt : 0, sp, d,
                                     best, best_id);
             if (own) {
                 leaves += pending;

@@ -36,6 +36,7 @@ struct DevScene {
     const float4 *tri_recs;   // the records once per triangle (indirect form)
     const int32_t *leaf_ids;  // record number per leaf entry (indirect form)
     int32_t n_entries;        // leaf entries (length of tris and leaf_ids; at least 1)
+    int32_t n_recs;           // per-triangle records (length of tri_recs; at least 1)
     const float4 *points;
     const float4 *normals;
     int32_t n_normals;
@@ -59,7 +60,9 @@ struct DevScene {
                         // bits 3-4 (with bit 2) leaf queue depth 1 << value,
                         // bit 5 six-wave register budget, bit 6 keyed leaf fold,
                         // bit 7 per-triangle records through leaf ids,
-                        // bits 8-9 (with bit 0) node window of 2 + value
+                        // bits 8-9 (with bit 0) node window of 2 + value,
+                        // bit 10 window cut at the 128-byte line end,
+                        // bit 11 buffer loads (fetch_fits)
 };
 
 struct Paths {

@@ -57,10 +57,13 @@ inline bool valid_fetch(int f) {
     }
 }
 
-// The buffer-load forms address nodes by 32-bit byte offsets, the window's
-// slots past the last node included: node arrays under 2 GiB.
-inline bool fetch_fits(int f, int64_t n_nodes) {
-    return !(f & 2048) || (n_nodes + 4) * 32 <= (int64_t)INT32_MAX;
+// The buffer-load forms address nodes, leaf entries' record numbers and
+// per-triangle records by 32-bit byte offsets (the window's slots past the
+// last node included): each array under 2 GiB.
+inline bool fetch_fits(int f, const DevScene &ds) {
+    return !(f & 2048) || ((ds.n_nodes + 4LL) * 32 <= (int64_t)INT32_MAX &&
+                           ds.n_entries * 4LL <= (int64_t)INT32_MAX &&
+                           ds.n_recs * 48LL <= (int64_t)INT32_MAX);
 }
 
 }  // namespace hpt

@@ -332,7 +332,7 @@ hpt_tuning tune(hpt_scene *s, Measure measure, bool shade) {
         for (int v : values) {
             hpt_tuning t = start;
             set(t, v);
-            if (!valid_fetch(t.fetch) || !fetch_fits(t.fetch, s->n_nodes)) continue;
+            if (!valid_fetch(t.fetch) || !fetch_fits(t.fetch, s->dev)) continue;
             // A knob moves only for a gain above the run-to-run noise.
             const double ms = timed(t, true);
             if (ms < best_ms * 0.99) {

@@ -179,8 +179,9 @@ int hpt_scene_stats_get(const hpt_scene *scene, hpt_scene_stats *out);
 
 /* Launch and arithmetic choices of a scene's kernels.  None changes a
  * result: every setting gives the same bits.  Defaults are set by
- * hpt_scene_create (fast_division = 1, traversal = 2, refill_threshold = 40,
- * node_steps = 24, fetch = 3517 (1469 for node arrays of 2 GiB or more),
+ * hpt_scene_create (fast_division = 1, traversal = 2, refill_threshold = 32,
+ * node_steps = 24, fetch = 3517 (1469 for scenes whose node, leaf-entry or
+ * per-triangle record array is 2 GiB or more),
  * walk_min = 0, sort_rays = 0, cast grid =
  * min(6, occupancy) x CUs, shade grid = occupancy x CUs). */
 typedef struct {
@@ -212,8 +213,9 @@ typedef struct {
                                     nodes, and bit 10 ends it at the end
                                     of the 128-byte line (four nodes) that
                                     holds the first; bit 11 reads the nodes
-                                    with buffer loads (node arrays under
-                                    2 GiB only).  The compiled forms: 0-5,
+                                    (and with bit 7 the leaf entries' ids and
+                                    the records) with buffer loads (arrays
+                                    under 2 GiB only).  The compiled forms: 0-5,
                                     12, 13, 20, 21, 28, 29, 52, 60, 84, 92,
                                     116, 124, 148, 149, 156, 157, 189, 220,
                                     252, 413, 445, 669, 701, 1469, 1725,
