@@ -79,9 +79,10 @@ __device__ __forceinline__ bool box_hit(V3 bs, V3 be, V3 sp, V3 d) {
 //    and that rare case is left to the division form.
 //  * x -> x -+ e|x| is monotone for e < 1, so the entry (max of mins) and
 //    exit formed from q' lie within relative e of those formed from q.
-//    When they differ by more than 2^-16 of their magnitudes (far above
-//    2 * 2^-21.9 plus the rounding of the check itself), their order is the
-//    exact order.  Otherwise the division form decides.
+//    When they differ by at least kCertify = 2^-20 of the sum of their
+//    magnitudes (3.7 times 2^-21.9, the rounding of the check itself and a
+//    subnormal bound included), their order is the exact order.
+//    Otherwise the division form decides.
 __device__ __forceinline__ bool fast_box_ray(V3 sp, V3 d) {
     auto dir_ok = [](float c) {
         const float a = fabsf(c);
@@ -99,6 +100,9 @@ __device__ __forceinline__ V3 approx_rcp(V3 d) {
     return mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
 }
 
+// Relative margin of the certified comparison (see above).
+constexpr float kCertify = 0x1p-20f;
+
 //
 // The certified part alone: the reciprocal form's decision, and in `open`
 // whether it is not certified (the caller then asks box_hit).
@@ -108,7 +112,7 @@ __device__ __forceinline__ bool box_hit_rcp_part(V3 bs, V3 be, V3 sp, V3 y, bool
     const float tz0 = (bs.z - sp.z) * y.z, tz1 = (be.z - sp.z) * y.z;
     const float entry = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     const float exit = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-    const float bound = (fabsf(entry) + fabsf(exit)) * 0x1p-16f;
+    const float bound = (fabsf(entry) + fabsf(exit)) * kCertify;
     const bool sure = fabsf(entry - exit) >= bound;
     const bool ahead = exit > 0.f;
     open = exit == 0.f || (ahead && !sure);

@@ -2,9 +2,9 @@
 hptracing_amd/csrc/device/dmath.hpp) as a method, on the CPU: quotients
 formed as (p - s) * y with y within 1 ulp of 1/d (here the correctly rounded
 reciprocal moved by -1, 0 or +1 ulp at random, the accuracy of v_rcp_f32),
-entry/exit compared with the 2^-16 relative margin, no start-inside test, and
-the division form where the margin does not certify, where the exit is 0, and
-for rays with a zero direction component.  Every decision must equal the division
+entry/exit compared with the 2^-20 relative margin, no start-inside test,
+and the division form where the margin does not certify, where the exit is
+0, and for rays with a zero direction component.  Every decision must equal the division
 form's (the slab test of kernel.cl:51-83, restated by the oracle) on
 adversarial rays aimed at box corners, edges and faces, and the fallback
 must be rare on random rays.  The GPU kernels themselves are checked against the oracle by
@@ -31,15 +31,23 @@ def box_div(bs, be, sp, d):
     return inside | ((exit_ > 0) & (entry <= exit_))
 
 
-def box_rcp(bs, be, sp, d, y):
+MARGIN = F(2.0 ** -20)
+
+
+def _certified(a, b, margin):
+    """|b - a| is at least `margin` of |a| + |b| (as the kernel computes it)."""
+    return np.abs(b - a) >= (np.abs(a) + np.abs(b)) * margin
+
+
+def box_rcp(bs, be, sp, d, y, margin=MARGIN):
     """The certified form: no start-inside test (a start point inside the
     box makes entry <= 0 <= exit, so only exit == 0 needs it), and rays with
     a zero direction component take the division form throughout, as
-    fast_box_ray sends them there."""
+    fast_box_ray sends them there.  Returns the decisions and which tests
+    the division form decided."""
     with np.errstate(all="ignore"):
         _, entry, exit_ = _slab(bs, be, sp, (bs - sp) * y, (be - sp) * y)
-        bound = (np.abs(entry) + np.abs(exit_)) * F(2.0 ** -16)
-        sure = np.abs(entry - exit_) >= bound
+        sure = _certified(entry, exit_, margin)
     ahead = exit_ > 0
     fast = ahead & (entry <= exit_)
     need = (exit_ == 0) | (ahead & ~sure) | np.any(d == 0, axis=1)
@@ -149,3 +157,19 @@ def test_start_inside_without_inside_test():
         _, entry, exit_ = _slab(lo, hi, org, (lo - org) * y, (hi - org) * y)
     no_fallback = (exit_ > 0) & (entry <= exit_)
     assert (no_fallback != ref).sum() > 0
+
+
+def test_margin_is_four_times_the_failing_one():
+    """On the adversarial near-ties the model's quotients (reciprocal within
+    1.5 ulp, then rounded) stay exact down to a margin of 2^-22; at 2^-23,
+    8 times tighter than the kernel's 2^-20, decisions differ from the
+    division form.  So the margin sits a factor 4 above where it fails, as
+    the error bound (2^-21.9 per quotient) says."""
+    rng = np.random.default_rng(12)
+    lo, hi, org, d = _rays(500_000, rng)
+    y = _approx_rcp(d, rng)
+    ref = box_div(lo, hi, org, d)
+    got, _ = box_rcp(lo, hi, org, d, y, margin=F(2.0 ** -22))
+    assert np.array_equal(got, ref)
+    got, _ = box_rcp(lo, hi, org, d, y, margin=F(2.0 ** -23))
+    assert (got != ref).sum() > 1000
