@@ -245,6 +245,13 @@ COUNTERS_FILE = os.path.join(ROOT, "profiles", "cast_counters_full1080p.json")
 CAST_KERNEL = "k_cast_refill"
 SQ_SET = ("SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU "
           "SQ_THREAD_CYCLES_VALU SQ_INSTS_SALU SQ_WAIT_INST_ANY")
+# The vector memory path of each CU: texture addresser (TA) and data (TD)
+# busy and stalled on the L1 (TCP), the L1's stalls on data pending from
+# the L2, and its L2 read requests with their summed latency, which give
+# the mean number of L2 reads in flight per CU (Little's law).
+MEM_SET = ("TA_TA_BUSY_sum TA_ADDR_STALLED_BY_TC_CYCLES_sum TD_TD_BUSY_sum TD_TC_STALL_sum "
+           "TCP_PENDING_STALL_CYCLES_sum TCP_TCC_READ_REQ_LATENCY_sum TCP_TCC_READ_REQ_sum")
+N_CU = 256
 
 
 def device_source_sha():
@@ -367,13 +374,29 @@ def summarize_counters(directory):
         "salu_per_valu": counters["SQ_INSTS_SALU"] / counters["SQ_INSTS_VALU"],
         "l2_hit": l2_hit,
     }
+    mem_dir = os.path.join(directory, "mem")
+    if os.path.isdir(mem_dir):
+        mem = _counter_rows(mem_dir, CAST_KERNEL)
+        m = {k: _avg(mem, k) for k in MEM_SET.split()}
+        counters.update(m)
+        cu_cycles = N_CU * cycles  # the _sum counters add one instance per CU
+        derived.update({
+            "ta_busy": m["TA_TA_BUSY_sum"] / cu_cycles,
+            "ta_addr_stalled_by_l1": m["TA_ADDR_STALLED_BY_TC_CYCLES_sum"] / cu_cycles,
+            "td_busy": m["TD_TD_BUSY_sum"] / cu_cycles,
+            "td_stalled_on_l1": m["TD_TC_STALL_sum"] / cu_cycles,
+            "l1_pending_stall": m["TCP_PENDING_STALL_CYCLES_sum"] / cu_cycles,
+            "l2_reads_per_launch": m["TCP_TCC_READ_REQ_sum"],
+            "l2_read_latency_cycles": m["TCP_TCC_READ_REQ_LATENCY_sum"] / m["TCP_TCC_READ_REQ_sum"],
+            "l2_reads_in_flight_per_cu": m["TCP_TCC_READ_REQ_LATENCY_sum"] / cu_cycles,
+        })
     rec = {
         "kernel": sq[0]["kernel"],
         "config": cfg,
         "commands": {
             "counters": "rocprofv3 --pmc <FETCH_SIZE | WRITE_SIZE | " + SQ_SET + " | "
-                        "TCC_HIT_sum TCC_MISS_sum> --output-format csv -d "
-                        "<DIR>/<fetch|write|sq|l2> -o run -- python bench.py "
+                        "TCC_HIT_sum TCC_MISS_sum | " + MEM_SET + "> --output-format csv -d "
+                        "<DIR>/<fetch|write|sq|l2|mem> -o run -- python bench.py "
                         "--steps 4 --warmup 0 --profile-steps 4 --no-cpu --batch-views 0 "
                         "--serve-requests 0 --cast-tris 0 --literal-steps 0 --profile-record <DIR>",
             "calibration": "rocprofv3 --pmc <FETCH_SIZE | WRITE_SIZE> --output-format csv "
@@ -504,6 +527,36 @@ def roofline_record(cast_bytes, per_launch_ms, tst, cst, rec, rec_source):
     lane_insts = sq["SQ_INSTS_VALU"] * d["active_lanes_of_64"]
     visits = int(cst.node_visits) / launches
     tests = int(cst.tri_tests) / launches
+    memory = None
+    if "l2_reads_in_flight_per_cu" in d:
+        memory = {
+            "l2_reads_in_flight_per_cu": round(d["l2_reads_in_flight_per_cu"], 1),
+            "l2_read_latency_cycles": round(d["l2_read_latency_cycles"], 1),
+            "l2_reads_per_launch": int(d["l2_reads_per_launch"]),
+            "ta_busy": round(d["ta_busy"], 3),
+            "ta_addr_stalled_by_l1": round(d["ta_addr_stalled_by_l1"], 3),
+            "td_busy": round(d["td_busy"], 3),
+            "td_stalled_on_l1": round(d["td_stalled_on_l1"], 3),
+            "l1_pending_stall": round(d["l1_pending_stall"], 3),
+        }
+    if memory:
+        binding = (f"the L1's reads in flight to the L2: {memory['l2_reads_in_flight_per_cu']:.0f} "
+                   f"per CU on average (Little's law over {memory['l2_read_latency_cycles']:.0f}-cycle "
+                   f"reads), the texture address path stalled by the L1 "
+                   f"{memory['ta_addr_stalled_by_l1']:.2f} of cycles and its data path waiting "
+                   f"on it {memory['td_stalled_on_l1']:.2f}, so throughput is that concurrency over "
+                   f"the latency; VALU busy {d['valu_busy']:.2f} of cycles with "
+                   f"{d['active_lanes_of_64']:.1f} of 64 lanes active, HBM traffic "
+                   f"{tgbs / HBM_PEAK_GBS:.2f} of peak.  A third more node loads per trip (to "
+                   f"lines already in flight) cost 17 %; 4.5 % fewer VALU instructions 1.6 %")
+    else:
+        binding = (f"VALU issue and the latency of dependent random fetches below the L2 "
+                   f"together, neither alone: VALU busy {d['valu_busy']:.2f} of cycles with "
+                   f"{d['active_lanes_of_64']:.1f} of 64 lanes active, HBM traffic "
+                   f"{tgbs / HBM_PEAK_GBS:.2f} of peak in 64- and 128-byte random reads; "
+                   f"4.5 % fewer VALU instructions (12 % fewer SALU, 17 % fewer LDS) "
+                   f"shortened the launch 1.6 %, and 23 % fewer node-fetch trips per ray "
+                   f"1.6 %")
     out.update({
         "traffic": traffic,
         "traffic_gbs": round(tgbs, 1),
@@ -517,13 +570,8 @@ def roofline_record(cast_bytes, per_launch_ms, tst, cst, rec, rec_source):
             "salu_per_valu": round(d["salu_per_valu"], 3),
             "lane_valu_insts_per_node_visit_or_test": round(lane_insts / (visits + tests), 1),
         },
-        "binding": (f"VALU issue and the latency of dependent random fetches below the L2 "
-                    f"together, neither alone: VALU busy {d['valu_busy']:.2f} of cycles with "
-                    f"{d['active_lanes_of_64']:.1f} of 64 lanes active, HBM traffic "
-                    f"{tgbs / HBM_PEAK_GBS:.2f} of peak in 64- and 128-byte random reads; "
-                    f"4.5 % fewer VALU instructions (12 % fewer SALU, 17 % fewer LDS) "
-                    f"shortened the launch 1.6 %, and 23 % fewer node-fetch trips per ray "
-                    f"1.6 %"),
+        "memory": memory,
+        "binding": binding,
     })
     return out
 
