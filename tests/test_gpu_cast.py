@@ -171,6 +171,38 @@ def test_cast_exact_ties(oracle_mod, fast):
     assert (ids >= 0).mean() > 0.05
 
 
+@pytest.fixture(scope="module")
+def tiny_scenes():
+    """Scenes of 1 to 13 large triangles (vertices uniform in the unit cube):
+    KD trees of 1 node (the root a leaf) up to a few hundred, so that node
+    windows and queued leaves reach past the last node and entry, and rays
+    aimed into the cube from around it."""
+    from hptracing_amd.scene import KDTree, material
+    out = []
+    for k in (1, 2, 3, 4, 5, 7, 9, 13):
+        rng = np.random.default_rng(40 + k)
+        pts = rng.uniform(0, 1, (3 * k, 3)).astype(np.float32)
+        mesh = np.concatenate([np.arange(3 * k, dtype=np.int32).reshape(k, 3),
+                               np.zeros((k, 1), np.int32)], 1)
+        kd = KDTree.from_arrays(pts, mesh, np.array([material(kd=(0.5, 0.5, 0.5))]))
+        tgt = rng.uniform(0, 1, (20000, 3)).astype(np.float32)
+        org = (tgt + rng.normal(size=(20000, 3)).astype(np.float32) * 2).astype(np.float32)
+        out.append((kd, org, (tgt - org).astype(np.float32)))
+    return out
+
+
+def test_cast_tiny_scenes(tiny_scenes, oracle_mod, fast):
+    """Trees of 1 to a few hundred nodes under every launch form: the node
+    window's slots past the last node (ordinary loads clamp them, buffer
+    loads read zeros there) are never visited, and ids and t equal the
+    oracle's."""
+    hits = 0
+    for kd, org, d in tiny_scenes:
+        ids, _ = _check(kd, org, d, oracle_mod, fast=fast)
+        hits += int((ids >= 0).sum())
+    assert hits > 40000
+
+
 def test_autotune_cast_keeps_results(soup_small):
     """hpt_autotune_cast picks launch forms on a ray batch; none changes a bit."""
     import torch
