@@ -70,6 +70,28 @@ def full_small(scene_dir):
     return KDTree(p, os.path.dirname(p) + "/")
 
 
+@pytest.mark.parametrize("kw", [
+    dict(depth=1),
+    dict(sample_x=1, sample_y=1, sample=1),
+    dict(sample_x=3, sample_y=5, sample=3, depth=2),
+    dict(sample_x=17, sample_y=9, sample=33, depth=3),
+    dict(right_dir=(1.0, 0.0, 0.0), sample_x=16, sample_y=16, sample=2),
+], ids=["depth-1", "one-pixel", "odd-3x5-depth-2", "33-spp-over-two-launch-groups", "all-miss"])
+def test_render_edge_cases(room, oracle_mod, kw):
+    """Shapes at the edges of the pass loop: no bounce level, one pixel and
+    one sample, odd sizes, a sample count that leaves a one-pass remainder
+    after two sixteen-pass launch groups, and a camera facing away from the
+    room (every path ends at the first cast; each pixel accumulates the
+    background once per sample).  Paths equal the oracle's."""
+    frac, rgba, ref_rgba, live, ref_live, acc, ref_acc = _compare(room, oracle_mod, **kw)
+    assert frac >= 0.99, frac
+    assert np.array_equal(live, np.asarray(ref_live)[: len(live)])
+    if "right_dir" in kw:
+        assert live[0] == 256 and not live[1:].any()
+        bg = np.float32([0.1, 0.2, 0.3]) * np.float32(2)
+        assert np.array_equal(acc, np.broadcast_to(bg, acc.shape))
+
+
 def test_render_full_scene_matches_oracle_paths(full_small, oracle_mod):
     """Every material kind of the 1080p scene (texture fetch, specular,
     refraction, diffuse, light) in the 1080p view with -x and depth 8,
