@@ -48,6 +48,39 @@ def test_batch_equals_solo(runner):
         assert np.array_equal(img0, img)
 
 
+def _edge_views():
+    return [
+        _view(sample_x=1, sample_y=1, sample=1, seed=21),
+        _view(sample_x=24, sample_y=8, sample=2, seed=22, depth=1),
+        _view(sample_x=16, sample_y=16, sample=2, seed=23, right_dir=(1.0, 0.0, 0.0)),
+        _view(sample_x=9, sample_y=7, sample=33, seed=24, depth=3),
+        _view(sample_x=40, sample_y=30, sample=17, seed=25),
+    ]
+
+
+def test_batch_edge_views_equal_solo(runner):
+    """A batch mixing one pixel, depth 1, a view whose rays all miss, and
+    sample counts that end in different launch groups (33 and 17 against
+    1 and 2): each view equals its stand-alone render bit for bit, and so
+    does every request of the same views on the server."""
+    from hptracing_amd.runner import RenderServer
+    views = _edge_views()
+    solo = _solo(runner, views)
+    imgs, accs = runner.render_batch(views, return_accum=True)
+    for (img0, acc0), img, acc in zip(solo, imgs, accs):
+        assert np.array_equal(acc0.view(np.int32), acc.view(np.int32))
+        assert np.array_equal(img0, img)
+    srv = RenderServer(runner, device_budget_bytes=64 << 20)
+    try:
+        ids = [srv.submit(v, return_accum=True) for v in views]
+        for (img0, acc0), rid in zip(solo, ids):
+            img, acc, _ = srv.wait(rid)
+            assert np.array_equal(acc0.view(np.int32), acc.view(np.int32))
+            assert np.array_equal(img0, img)
+    finally:
+        srv.close()
+
+
 def test_graph_and_direct_launches_agree(runner):
     views = _views()
     a = runner.render_batch(views, return_accum=True, use_graph=True)[1]
