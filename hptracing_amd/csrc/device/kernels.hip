@@ -243,27 +243,6 @@ __device__ __forceinline__ float4 buffer_float4(__amdgpu_buffer_rsrc_t r, uint32
                        __int_as_float(v[3]));
 }
 
-// The three float4 of leaf entry `at`'s record (tri_record); BUF reads the
-// indirect form's record number and record with buffer loads.
-template <bool IND, bool BUF>
-__device__ __forceinline__ void load_record(const DevScene &sc, int at, float4 &r0, float4 &r1,
-                                            float4 &r2) {
-    if constexpr (IND && BUF) {
-        const int k = __builtin_amdgcn_raw_buffer_load_b32(
-            buffer_over(sc.leaf_ids, (uint32_t)sc.n_entries * 4u), (int)((uint32_t)at << 2), 0, 0);
-        const __amdgpu_buffer_rsrc_t recs = buffer_over(sc.tri_recs, (uint32_t)sc.n_recs * 48u);
-        const uint32_t off = (uint32_t)k * 48u;
-        r0 = buffer_float4(recs, off);
-        r1 = buffer_float4(recs, off + 16u);
-        r2 = buffer_float4(recs, off + 32u);
-    } else {
-        const float4 *rec = tri_record<IND>(sc, at);
-        r0 = rec[0];
-        r1 = rec[1];
-        r2 = rec[2];
-    }
-}
-
 // Leaf entry of position k (0 <= k < the lane's count) of the work of the
 // lane whose queue is in column `col`, with `nq` leaves queued: the slots
 // from nq on hold stale or never-written values and do not take part.  The
@@ -281,14 +260,24 @@ __device__ __forceinline__ int queue_record(const LeafQueue<Q> &lq, int col, int
     return min(max(at, 0), n_entries - 1);
 }
 
+// Each lane's mailbox of the record numbers its ray has tested (IND and BUF
+// forms): slot k & 3 holds the last record number k tested there, and -1
+// when the lane takes a ray.  A test of a record its ray has already tested
+// gives the same distance again and cannot change the result under the
+// strict `r < best` fold (the earlier test set best <= r, or lost), so it
+// is skipped: no record loads, no test.  Skipping only later copies keeps
+// the first smallest distance, so ties come out as the reference's.
+constexpr int kMailbox = 4;
+
 // One leaf round of the FLAT form (see cast_refill).  A lane's pending work
 // is the triangles of its `nq` queued leaves in queue order, `cnt` in all
 // (0 and 0 if none); best / best_id are folded in place.  All 64 lanes must
-// call it.
+// call it.  `mb` is the block's mailboxes (IND and BUF forms).
 template <int Q, bool IND, bool BUF>
 __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
                                                 int lane, int nq, int cnt, V3 sp, V3 d,
-                                                float &best, int &best_id) {
+                                                float &best, int &best_id,
+                                                int (*mb)[kBlock]) {
     __shared__ int s_mark[kBlock];
     const int w = threadIdx.x & ~63;
     const int incl = wave_scan_add(cnt, lane);
@@ -317,11 +306,34 @@ __device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQu
             // Position j - o_excl of the owner's work (0 <= it < the owner's
             // count, the sum of the counts of its o_nq queued leaves).
             const int at = queue_record<Q>(lq, w + own, o_nq, j - o_excl, sc.n_entries);
-            float4 r0, r1, r2;
-            load_record<IND, BUF>(sc, at, r0, r1, r2);
-            r = sc.arith_ocl ? tri_hit_ocl(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2))
-                             : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
-            id = __float_as_int(r0.w);
+            if constexpr (IND && BUF) {
+                // The owner's mailbox is read by every position of the chunk
+                // before any of them writes it, so a repeat inside one chunk
+                // is tested again (harmless); repeats of earlier chunks and
+                // rounds are skipped.
+                const int k = __builtin_amdgcn_raw_buffer_load_b32(
+                    buffer_over(sc.leaf_ids, (uint32_t)sc.n_entries * 4u),
+                    (int)((uint32_t)at << 2), 0, 0);
+                int &slot = mb[k & (kMailbox - 1)][w + own];
+                if (slot != k) {
+                    const __amdgpu_buffer_rsrc_t recs =
+                        buffer_over(sc.tri_recs, (uint32_t)sc.n_recs * 48u);
+                    const uint32_t off = (uint32_t)k * 48u;
+                    const float4 r0 = buffer_float4(recs, off);
+                    const float4 r1 = buffer_float4(recs, off + 16u);
+                    const float4 r2 = buffer_float4(recs, off + 32u);
+                    r = sc.arith_ocl ? tri_hit_ocl(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2))
+                                     : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
+                    id = __float_as_int(r0.w);
+                    slot = k;
+                }
+            } else {
+                const float4 *rec = tri_record<IND>(sc, at);
+                const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+                r = sc.arith_ocl ? tri_hit_ocl(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2))
+                                 : tri_hit(o_sp, o_d, xyz(r0), xyz(r1), xyz(r2));
+                id = __float_as_int(r0.w);
+            }
         }
         // Fold the chunk per owner: a segment is one owner's run of
         // positions, starting at its first position or at the chunk start.
@@ -455,6 +467,8 @@ __device__ __forceinline__ void keyed_leaf_round(const DevScene &sc, const LeafQ
 //         64-bit address arithmetic) and a range check that returns zeros
 //         past the last node (such window slots are never visited), so the
 //         window needs no clamp.  Each array must be under 2 GiB (fetch_fits).
+//         With FLAT and IND, leaf rounds skip records the ray has already
+//         tested (the mailbox, kMailbox).
 template <bool PATHS, bool FAST, int WIN, bool PREF, int Q, bool KEYED, bool IND, bool LINE,
           bool BUF>
 __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org3,
@@ -464,6 +478,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
     constexpr bool FLAT = Q > 0;
     constexpr int QN = Q > 0 ? Q : 1;
     __shared__ LeafQueue<QN> lq;
+    __shared__ int s_mb[(FLAT && IND && BUF) ? kMailbox : 1][kBlock];
     __shared__ RaySlots rs;
     __shared__ unsigned long long s_key[kBlock];
     const int lane = threadIdx.x & 63;
@@ -584,6 +599,10 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                     pcnt = 0;
                     pending = 0;
                     done = nn <= 0;
+                    if constexpr (FLAT && IND && BUF) {
+#pragma unroll
+                        for (int m = 0; m < kMailbox; ++m) s_mb[m][tid] = -1;
+                    }
                 }
             }
         }
@@ -648,7 +667,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                                      own ? pcnt : 0, best, best_id);
             else
                 flat_leaf_round<QN, IND, BUF>(sc, lq, lane, own ? pending : 0, own ? pcnt : 0, sp, d,
-                                    best, best_id);
+                                              best, best_id, s_mb);
             if (own) {
                 leaves += pending;
                 if (leaves >= kMaxMatchData) done = true;

@@ -181,3 +181,104 @@ def test_segmented_fold_is_the_sequential_fold_per_segment():
                 assert not (fr[l] > 0), (trial, l)
                 continue
             assert fr[l] == b and fi[l] == bi, (trial, l)
+
+
+MAILBOX = 4
+
+
+def mailbox_rounds(work, dist, rays_of, clear_on_new_ray=True, mark_before_test=False):
+    """The leaf rounds of the buffer forms with the per-ray mailbox
+    (flat_leaf_round in kernels.hip): `work[r]` is round r's list of
+    (lane, record) positions in position order, `rays_of[r][lane]` the ray
+    the lane holds in round r and `dist(ray, record)` the test's distance.
+    Chunk by chunk, every position first reads its owner's mailbox slot
+    record & 3; positions whose slot holds their record are skipped, the
+    others are tested and then write their record there.  The mailbox is
+    cleared when a lane takes a new ray.  Returns each ray's (best, id) and
+    the number of skipped tests."""
+    mb = {}
+    best = {}
+    skipped = 0
+    held = {}
+    for r, positions in enumerate(work):
+        for lane, ray in rays_of[r].items():
+            if held.get(lane) != ray:
+                held[lane] = ray
+                if clear_on_new_ray:
+                    mb[lane] = [-1] * MAILBOX
+                else:
+                    mb.setdefault(lane, [-1] * MAILBOX)
+                best[ray] = (-42.0, -1)
+        for base in range(0, len(positions), WAVE):
+            chunk = positions[base:base + WAVE]
+            if mark_before_test:  # a planted mistake: slots written before they are read
+                for lane, k in chunk:
+                    mb[lane][k % MAILBOX] = k
+            seen = [mb[lane][k % MAILBOX] == k for lane, k in chunk]
+            rs, ids, owners = [], [], []
+            for (lane, k), skip in zip(chunk, seen):
+                ray = rays_of[r][lane]
+                owners.append(ray)
+                if skip:
+                    skipped += 1
+                    rs.append(-1.0)
+                else:
+                    rs.append(dist(ray, k))
+                    mb[lane][k % MAILBOX] = k
+                ids.append(k)
+            for ray in dict.fromkeys(owners):
+                sel = [i for i, o in enumerate(owners) if o == ray]
+                b, bi = seq_fold(*best[ray], [rs[i] for i in sel], [ids[i] for i in sel])
+                best[ray] = (b, bi)
+    return best, skipped
+
+
+def test_mailbox_skips_only_repeats_of_the_same_ray():
+    """Random rounds of a wave: lanes keep rays for several rounds and then
+    take new ones, each ray's records repeat often (a triangle listed in
+    many leaves), distances are drawn from a few values so that different
+    records tie exactly, and some are misses (<= 0) or infinite.  With the
+    mailbox, every ray's (best, id) equals the reference's sequential fold
+    over all its tests; many tests are skipped.  Planted mistakes fail: a
+    mailbox kept across rays skips a new ray's first tests, and one written
+    before it is read skips first copies."""
+    rng = np.random.default_rng(23)
+    n_rounds, lanes = 40, 64
+    values = np.float32([0.5, 0.75, 1.0, 1.0, 2.5, -1.0, 0.0, np.inf])
+    table = {}
+
+    def dist(ray, k):
+        if (ray, k) not in table:
+            table[(ray, k)] = float(values[rng.integers(0, len(values))])
+        return table[(ray, k)]
+
+    rays_of, work, all_tests = [], [], {}
+    next_ray, cur = 0, {}
+    for r in range(n_rounds):
+        for lane in range(lanes):
+            if lane not in cur or rng.random() < 0.15:
+                cur[lane] = next_ray
+                next_ray += 1
+        rays_of.append(dict(cur))
+        positions = []
+        for lane in range(lanes):
+            if rng.random() < 0.3:
+                continue
+            pool = rng.integers(0, 12, 3)  # a few records a ray meets again and again
+            n = int(rng.integers(1, 40))
+            recs = np.where(rng.random(n) < 0.6, rng.choice(pool, n), rng.integers(0, 5000, n))
+            for k in recs:
+                positions.append((lane, int(k)))
+                all_tests.setdefault(cur[lane], []).append(int(k))
+        work.append(positions)
+    got, skipped = mailbox_rounds(work, dist, rays_of)
+    for ray, recs in all_tests.items():
+        want = seq_fold(-42.0, -1, [dist(ray, k) for k in recs], recs)
+        assert got[ray] == want, (ray, got[ray], want)
+    assert skipped > 0.05 * sum(len(v) for v in all_tests.values())
+    kept, _ = mailbox_rounds(work, dist, rays_of, clear_on_new_ray=False)
+    assert any(kept[ray] != seq_fold(-42.0, -1, [dist(ray, k) for k in recs], recs)
+               for ray, recs in all_tests.items())
+    early, _ = mailbox_rounds(work, dist, rays_of, mark_before_test=True)
+    assert any(early[ray] != seq_fold(-42.0, -1, [dist(ray, k) for k in recs], recs)
+               for ray, recs in all_tests.items())
