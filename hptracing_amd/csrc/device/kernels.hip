@@ -216,10 +216,12 @@ __device__ __forceinline__ void wave_seg_fold(int lane, int seg_start, float &r,
 // triangle in that work (the counts of the leaves before it) and base[q]
 // its first leaf entry minus start[q], so that position k of the lane's
 // work is leaf entry base[q] + k for the last q with start[q] <= k.
-template <int Q>
+template <int Q, int ROWS = Q + 1>
 struct LeafQueue {
-    int base[Q + 1][kBlock];   // slot Q takes the writes of lanes that queue nothing
-    int start[Q + 1][kBlock];
+    // With ROWS = Q + 1, slot Q takes the writes of lanes that queue nothing
+    // (the single-node reciprocal visit); the window forms need Q rows.
+    int base[ROWS][kBlock];
+    int start[ROWS][kBlock];
 };
 
 // The record of leaf entry `at`: in leaf order (direct form), or through
@@ -250,8 +252,8 @@ __device__ __forceinline__ float4 buffer_float4(__amdgpu_buffer_rsrc_t r, uint32
 // so the leaf holding k is the number of later slots starting at or before
 // k.  The entry is clamped into the scene's leaf lists, so that no error in
 // the bookkeeping can read outside them.
-template <int Q>
-__device__ __forceinline__ int queue_record(const LeafQueue<Q> &lq, int col, int nq, int k,
+template <int Q, int R>
+__device__ __forceinline__ int queue_record(const LeafQueue<Q, R> &lq, int col, int nq, int k,
                                             int n_entries) {
     int q = 0;
 #pragma unroll
@@ -261,20 +263,20 @@ __device__ __forceinline__ int queue_record(const LeafQueue<Q> &lq, int col, int
 }
 
 // Each lane's mailbox of the record numbers its ray has tested (IND and BUF
-// forms): slot k & 3 holds the last record number k tested there, and -1
+// forms): slot k & 7 holds the last record number k tested there, and -1
 // when the lane takes a ray.  A test of a record its ray has already tested
 // gives the same distance again and cannot change the result under the
 // strict `r < best` fold (the earlier test set best <= r, or lost), so it
 // is skipped: no record loads, no test.  Skipping only later copies keeps
 // the first smallest distance, so ties come out as the reference's.
-constexpr int kMailbox = 4;
+constexpr int kMailbox = 8;
 
 // One leaf round of the FLAT form (see cast_refill).  A lane's pending work
 // is the triangles of its `nq` queued leaves in queue order, `cnt` in all
 // (0 and 0 if none); best / best_id are folded in place.  All 64 lanes must
 // call it.  `mb` is the block's mailboxes (IND and BUF forms).
-template <int Q, bool IND, bool BUF>
-__device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQueue<Q> &lq,
+template <int Q, bool IND, bool BUF, int R>
+__device__ __forceinline__ void flat_leaf_round(const DevScene &sc, const LeafQueue<Q, R> &lq,
                                                 int lane, int nq, int cnt, V3 sp, V3 d,
                                                 float &best, int &best_id,
                                                 int (*mb)[kBlock]) {
@@ -477,7 +479,10 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
                                             int *out_id, float *out_t) {
     constexpr bool FLAT = Q > 0;
     constexpr int QN = Q > 0 ? Q : 1;
-    __shared__ LeafQueue<QN> lq;
+    // The window forms (WIN > 1) never use the reciprocal single-node
+    // visit, so their queue has no spare row.
+    constexpr int kSpare = WIN < 2 ? QN : 0;
+    __shared__ LeafQueue<QN, WIN < 2 ? QN + 1 : QN> lq;
     __shared__ int s_mb[(FLAT && IND && BUF) ? kMailbox : 1][kBlock];
     __shared__ RaySlots rs;
     __shared__ unsigned long long s_key[kBlock];
@@ -551,7 +556,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
         const bool leaf = b >= 0;
         const bool take = leaf && hit && a > 0;
         idx = (!leaf && !hit) ? a : idx + 1;
-        const int slot = take ? pending : QN;
+        const int slot = take ? pending : kSpare;
         lq.base[slot][tid] = b - pcnt;
         lq.start[slot][tid] = pcnt;
         pcnt += take ? a : 0;
@@ -662,7 +667,7 @@ __device__ __forceinline__ void cast_refill(const DevScene &sc, const float *org
         if (FLAT) {
             __builtin_amdgcn_wave_barrier();
             const bool own = ray >= 0 && pending > 0;
-            if (KEYED)
+            if constexpr (KEYED)
                 keyed_leaf_round<QN, IND>(sc, lq, rs, s_key, lane, own ? pending : 0,
                                      own ? pcnt : 0, best, best_id);
             else

@@ -183,7 +183,7 @@ def test_segmented_fold_is_the_sequential_fold_per_segment():
             assert fr[l] == b and fi[l] == bi, (trial, l)
 
 
-MAILBOX = 4
+MAILBOX = 8
 
 
 def mailbox_rounds(work, dist, rays_of, clear_on_new_ray=True, mark_before_test=False):
@@ -192,7 +192,7 @@ def mailbox_rounds(work, dist, rays_of, clear_on_new_ray=True, mark_before_test=
     (lane, record) positions in position order, `rays_of[r][lane]` the ray
     the lane holds in round r and `dist(ray, record)` the test's distance.
     Chunk by chunk, every position first reads its owner's mailbox slot
-    record & 3; positions whose slot holds their record are skipped, the
+    record & 7; positions whose slot holds their record are skipped, the
     others are tested and then write their record there.  The mailbox is
     cleared when a lane takes a new ray.  Returns each ray's (best, id) and
     the number of skipped tests."""
