@@ -62,7 +62,8 @@ struct DevScene {
                         // bit 7 per-triangle records through leaf ids,
                         // bits 8-9 (with bit 0) node window of 2 + value,
                         // bit 10 window cut at the 128-byte line end,
-                        // bit 11 buffer loads (fetch_fits)
+                        // bit 11 buffer loads (fetch_fits) and, with bits
+                        // 2 and 7, the per-ray record mailbox
 };
 
 struct Paths {

@@ -215,7 +215,9 @@ typedef struct {
                                     holds the first; bit 11 reads the nodes
                                     (and with bit 7 the leaf entries' ids and
                                     the records) with buffer loads (arrays
-                                    under 2 GiB only).  The compiled forms: 0-5,
+                                    under 2 GiB only) and, with bits 2 and 7,
+                                    skips triangle tests a ray has already
+                                    made.  The compiled forms: 0-5,
                                     12, 13, 20, 21, 28, 29, 52, 60, 84, 92,
                                     116, 124, 148, 149, 156, 157, 189, 220,
                                     252, 413, 445, 669, 701, 1469, 1725,
