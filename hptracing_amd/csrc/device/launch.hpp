@@ -39,12 +39,12 @@ int occupancy_blocks_shade();
 // depth 1, 2, 4 or 8 (bits 3-4 = 0..3; with 4 only, and without 2); with
 // a queue of 4 or 8, bit 5 (32) compiles the form for six waves per SIMD
 // and (without node pairs) bit 6 (64) folds the leaf rounds through keys in
-// LDS; bit 7 (128)
-// reads the triangle records once per triangle through the leaf entries'
-// ids; bits 8-9 (with 1) widen the node window, bit 10 (1024) ends it at
-// the end of the 128-byte line that holds the first node, and bit 11 (2048)
-// reads the nodes with buffer loads (fetch_fits).  These are the compiled
-// forms.
+// LDS; bit 7 (128) reads the triangle records once per triangle through
+// the leaf entries' ids; bits 8-9 (with 1) widen the node window, bit 10
+// (1024) ends it at the end of the 128-byte line that holds the first node,
+// and bit 11 (2048) reads the nodes, and with 4 and 128 the leaf entries'
+// ids and the records, with buffer loads (fetch_fits) and skips records a
+// ray has already tested (its mailbox).  These are the compiled forms.
 inline bool valid_fetch(int f) {
     switch (f) {
     case 0: case 1: case 2: case 3: case 4: case 5: case 12: case 13: case 20: case 21:
